@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the race-free ECL-CC path on one MI355X.
+
+Default (``python bench.py``): plain ECL-CC on R-MAT scale 24 (edge factor
+16, a,b,c = .57,.19,.19, symmetrised, deduplicated, IDs permuted, seed 24),
+CSR resident in HBM.  One step = one full CC pass (init, compute1, compute2,
+compute3, flatten).  Prints ONE JSON line with edges/s (edges = CSR entries,
+the reference's definition, ECL-CC-atomic.cu:316), the HBM roofline of the
+pass, and the CPU baseline (the oracle's threaded ECL-CC restatement on the
+host cores, same graph).
+
+Other modes: ``--mode batched`` (many graphs, one hipGraph launch, against
+one graph per launch) and ``--mode served`` (continuous batching under
+Poisson arrivals, p50/p99 latency).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "ecl-suite_amd"))
+sys.path.insert(0, str(REPO))
+
+METRIC = "CC edges/s + % HBM peak on RMAT-24; batched/served graphs/s with p50/p99 ms"
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from eclcc import workloads
+
+    if name == "rmat24":
+        return workloads.rmat24()
+    if name == "grid256":
+        return workloads.grid256()
+    if name == "er65536":
+        return workloads.er65536(1)
+    if name.startswith("rmat"):
+        return workloads.rmat(int(name[4:]))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(g, budget_s: float, gpu_labels):
+    """Threaded ECL-CC restatement (oracle/) on the host cores."""
+    import numpy as np
+    from oracle import oracle
+
+    cores = min(16, os.cpu_count() or 1)
+    times = []
+    t_end = time.perf_counter() + budget_s
+    labels = None
+    while len(times) < 9 and (not times or time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        labels, used = oracle.eclcc_threads(g.nindex, g.nlist, cores)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {
+        "value": g.edges / med,
+        "unit": "edges/s",
+        "cores": used,
+        "kind": "port",
+        "sample": f"full {g.name} (n={g.nodes}, m={g.edges}), median of {len(times)} runs of the "
+                  f"oracle's threaded ECL-CC restatement",
+        "seconds": med,
+        "labels_match_gpu": bool(np.array_equal(labels, gpu_labels)),
+    }
+
+
+def plain(args) -> dict:
+    import torch
+    from eclcc import DeviceGraph, default_params
+    from eclcc.cc import KERNELS
+
+    t0 = time.perf_counter()
+    g = make_workload(args.workload)
+    log(f"generated {g.name}: n={g.nodes} m={g.edges} in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    dg = DeviceGraph(g, "cuda:0", default_params())
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        dg.run(stream)
+    torch.cuda.synchronize()
+    # per-kernel durations: HIP events between the five kernels on the launch stream
+    per = [[] for _ in KERNELS]
+    tots = []
+    for _ in range(max(3, min(args.steps, 10))):
+        kms, tot = dg.run_profiled(stream)
+        for i, x in enumerate(kms):
+            per[i].append(x)
+        tots.append(tot)
+    labels = dg.labels()
+    log("timed region")
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        dg.run(stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    if not (labels == dg.labels()).all():
+        raise SystemExit("labels changed between runs")
+    B = g.algorithmic_bytes()
+    kernel_ms = {k: statistics.mean(v) for k, v in zip(KERNELS, per)}
+    pass_ms = statistics.mean(tots)
+    hubs, tasks = dg.hub_stats()
+    out = {
+        "metric": METRIC,
+        "value": g.edges / (ms * 1e-3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": g.name, "nodes": g.nodes, "edges": g.edges,
+                   "algorithmic_bytes": B, "params": dg.params.as_dict()},
+        "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "ECL-CC pass (init+compute1+compute2+compute3+flatten)",
+            "achieved": B / (pass_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": B / (pass_ms * 1e-3) / HBM_PEAK,
+            "traffic": None,
+        },
+        "kernels_ms": kernel_ms,
+        "pass_ms_events": pass_ms,
+        "hub_vertices": hubs,
+        "hub_tasks": tasks,
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_budget, labels)
+    return out
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served"])
+    ap.add_argument("--workload", default="rmat24")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(graphs are independent units; see the sharding design). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "plain":
+        out = plain(args)
+    else:
+        from eclcc import benchmodes
+
+        out = benchmodes.run(args.mode, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

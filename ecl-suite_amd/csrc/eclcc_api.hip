@@ -1,0 +1,555 @@
+// eclcc_api.hip -- host side of libeclcc.so: the C ABI of include/eclcc.h for
+// the plain drop-in (computeCC, ECL-CC-atomic.cu:275-325), device-resident
+// runs and batches.  No function here exits the process: the reference's
+// exit(-1) paths (:280, :290-296, :319; ECLatomic.h:57-65) become statuses.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "eclcc_internal.h"
+
+namespace eclcc {
+
+namespace {
+thread_local std::string t_err;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+}  // namespace
+
+int set_error(int code, const std::string& msg) {
+  t_err = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, int code, const char* what) {
+  if (e == hipSuccess) return ECLCC_OK;
+  return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int device_cu_count() {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (cache[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
+int resolve_tuning(const eclcc_params* in, Tuning* out) {
+  eclcc_params d;
+  eclcc_params_default(&d);
+  const eclcc_params& p = in ? *in : d;
+  Tuning t;
+  t.thread_max = p.thread_max_deg;
+  t.wave_max = p.wave_max_deg;
+  t.hub_chunk = p.hub_chunk;
+  t.block_threads = p.block_threads;
+  t.hub_blocks = p.hub_blocks;
+  if (t.thread_max < 0 || t.wave_max < t.thread_max)
+    return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
+  if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
+  if (t.hub_chunk < 64 || t.hub_chunk > kMaxHubChunk)
+    return set_error(ECLCC_ERR_ARG, "hub_chunk must be in [64, 8192]");
+  if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
+    return set_error(ECLCC_ERR_ARG, "block_threads must be 256, 512 or 1024");
+  if (t.hub_blocks <= 0) {
+    const int cus = device_cu_count();
+    t.hub_blocks = 8 * (cus > 0 ? cus : 256);
+  }
+  *out = t;
+  return ECLCC_OK;
+}
+
+long long hub_capacity(long long edges, int wave_max, long long ngraphs) {
+  // every listed vertex has more than wave_max entries
+  return edges / ((long long)wave_max + 1) + ngraphs;
+}
+
+size_t workspace_bytes(long long nslots, long long hub_cap) {
+  size_t b = 256;                                              // counters, flags
+  b += align_up(sizeof(unsigned long long) * (size_t)((nslots + 63) / 64), 256);  // bitmap
+  b += align_up(sizeof(HubEntry) * (size_t)std::max<long long>(hub_cap, 1), 256);
+  return b;
+}
+
+bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_cap, Workspace* w) {
+  if (!base || bytes < workspace_bytes(nslots, hub_cap)) return false;
+  if ((reinterpret_cast<uintptr_t>(base) & 15) != 0) return false;
+  char* p = static_cast<char*>(base);
+  w->hubctr = reinterpret_cast<unsigned long long*>(p);
+  w->flags = reinterpret_cast<unsigned int*>(p + 128);
+  p += 256;
+  w->midbits = reinterpret_cast<unsigned long long*>(p);
+  p += align_up(sizeof(unsigned long long) * (size_t)((nslots + 63) / 64), 256);
+  w->hubs = reinterpret_cast<HubEntry*>(p);
+  w->hub_cap = std::max<long long>(hub_cap, 1);
+  return true;
+}
+
+}  // namespace eclcc
+
+using namespace eclcc;
+
+namespace {
+
+int check_csr_args(int32_t nodes, int64_t edges, const void* nindex, const void* nlist) {
+  if (nodes < 1 || edges < 0) return set_error(ECLCC_ERR_ARG, "node or edge count too low");
+  if (edges > INT32_MAX) return set_error(ECLCC_ERR_ARG, "edge count exceeds the int32 CSR format");
+  if (!nindex || (edges > 0 && !nlist)) return set_error(ECLCC_ERR_ARG, "null CSR pointer");
+  return ECLCC_OK;
+}
+
+KParams single_params(int32_t nodes, const int32_t* d_nindex, const int32_t* d_nlist,
+                      int32_t* d_nstat, const Workspace& w, const Tuning& t) {
+  KParams p{};
+  p.nidx = d_nindex;
+  p.nlist = d_nlist;
+  p.nstat = d_nstat;
+  p.n = nodes;
+  p.nslots = nodes;
+  p.desc = nullptr;
+  p.chunk_graph = nullptr;
+  p.midbits = w.midbits;
+  p.hubs = w.hubs;
+  p.hubctr = w.hubctr;
+  p.flags = w.flags;
+  p.hub_cap = w.hub_cap;
+  p.thread_max = t.thread_max;
+  p.wave_max = t.wave_max;
+  p.hub_chunk = t.hub_chunk;
+  return p;
+}
+
+// Device buffers of one plain call, freed on scope exit.
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  ~DeviceBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  int alloc(void** out, size_t bytes, const char* what) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(ECLCC_ERR_ALLOC, std::string("could not allocate ") + what + ": " +
+                                            hipGetErrorString(e));
+    }
+    ptrs.push_back(p);
+    *out = p;
+    return ECLCC_OK;
+  }
+};
+
+int overflow_check(const Workspace& w, hipStream_t s) {
+  unsigned int flags = 0;
+  int rc = check_hip(hipMemcpyAsync(&flags, w.flags, sizeof flags, hipMemcpyDeviceToHost, s),
+                     ECLCC_ERR_COPY, "copying from device failed");
+  if (rc) return rc;
+  rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_LAUNCH, "kernel execution failed");
+  if (rc) return rc;
+  if (flags & 1u) return set_error(ECLCC_ERR_OVERFLOW, "high-degree worklist overflow");
+  return ECLCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* eclcc_last_error(void) { return t_err.c_str(); }
+
+const char* eclcc_status_string(int s) {
+  switch (s) {
+    case ECLCC_OK: return "ok";
+    case ECLCC_ERR_ARG: return "invalid argument";
+    case ECLCC_ERR_ALLOC: return "allocation failed";
+    case ECLCC_ERR_COPY: return "copy failed";
+    case ECLCC_ERR_LAUNCH: return "kernel launch or execution failed";
+    case ECLCC_ERR_DEVICE: return "no usable HIP device";
+    case ECLCC_ERR_BUDGET: return "request exceeds the HBM budget";
+    case ECLCC_ERR_STATE: return "wrong state or unknown id";
+    case ECLCC_ERR_OVERFLOW: return "worklist overflow";
+    case ECLCC_ERR_TIMEOUT: return "timed out";
+    default: return "unknown status";
+  }
+}
+
+const char* eclcc_version(void) { return "eclcc-mi355x 0.1 (gfx950, wave64)"; }
+
+void eclcc_params_default(eclcc_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->thread_max_deg = 16;
+  p->wave_max_deg = 704;  // 11 x 64-lane wavefronts
+  p->hub_chunk = 4096;
+  p->hub_blocks = 0;
+  p->block_threads = 256;
+}
+
+size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) {
+  Tuning t;
+  if (nodes < 1 || edges < 0 || resolve_tuning(params, &t) != ECLCC_OK) return 0;
+  return workspace_bytes(nodes, hub_capacity(edges, t.wave_max, 1));
+}
+
+int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, const int32_t* d_nlist,
+                     int32_t* d_nstat, void* d_work, size_t work_bytes, const eclcc_params* params,
+                     void* stream) {
+  int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
+  if (rc) return rc;
+  if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
+  Tuning t;
+  if ((rc = resolve_tuning(params, &t))) return rc;
+  Workspace w;
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+    return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
+  const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
+  return check_hip(launch_pipeline(p, false, Shape{t.hub_blocks, t.block_threads},
+                                   (hipStream_t)stream, nullptr),
+                   ECLCC_ERR_LAUNCH, "kernel launch failed");
+}
+
+int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                              const int32_t* d_nlist, int32_t* d_nstat, void* d_work,
+                              size_t work_bytes, const eclcc_params* params, void* stream,
+                              float* kernel_ms, float* total_ms) {
+  int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
+  if (rc) return rc;
+  if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
+  Tuning t;
+  if ((rc = resolve_tuning(params, &t))) return rc;
+  Workspace w;
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+    return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
+  const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
+  hipEvent_t ev[6] = {};
+  for (int i = 0; i < 6; i++) {
+    if ((rc = check_hip(hipEventCreate(&ev[i]), ECLCC_ERR_DEVICE, "event create failed"))) {
+      for (int j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
+      return rc;
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  rc = check_hip(launch_pipeline(p, false, Shape{t.hub_blocks, t.block_threads}, s, ev),
+                 ECLCC_ERR_LAUNCH, "kernel launch failed");
+  if (!rc) rc = check_hip(hipEventSynchronize(ev[5]), ECLCC_ERR_LAUNCH, "kernel execution failed");
+  if (!rc) {
+    for (int k = 0; k < 5; k++) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      if (kernel_ms) kernel_ms[k] = ms;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
+    if (total_ms) *total_ms = tot;
+  }
+  for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+  if (!rc) rc = overflow_check(w, s);
+  return rc;
+}
+
+int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                        const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
+                        void* stream) {
+  int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
+  if (rc) return rc;
+  if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
+  Tuning t;
+  if ((rc = resolve_tuning(nullptr, &t))) return rc;
+  Workspace w;
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+    return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
+  const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
+  return check_hip(launch_init_only(p, (hipStream_t)stream), ECLCC_ERR_LAUNCH,
+                   "kernel launch failed");
+}
+
+int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, int64_t* n_hubs,
+                          int64_t* n_hub_tasks) {
+  if (!d_work || work_bytes < 256) return set_error(ECLCC_ERR_ARG, "bad workspace");
+  unsigned long long ctr = 0;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = check_hip(hipMemcpyAsync(&ctr, d_work, sizeof ctr, hipMemcpyDeviceToHost, s),
+                     ECLCC_ERR_COPY, "copying from device failed");
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_LAUNCH, "kernel execution failed");
+  if (rc) return rc;
+  if (n_hubs) *n_hubs = (int64_t)(unsigned)ctr;
+  if (n_hub_tasks) *n_hub_tasks = (int64_t)(ctr >> 32);
+  return ECLCC_OK;
+}
+
+int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
+                        int32_t* nstat, const eclcc_params* params, double* seconds) {
+  int rc = check_csr_args(nodes, edges, nindex, nlist);
+  if (rc) return rc;
+  if (!nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
+  Tuning t;
+  if ((rc = resolve_tuning(params, &t))) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return set_error(ECLCC_ERR_DEVICE, "there is no HIP capable device");
+  }
+  const long long cap = hub_capacity(edges, t.wave_max, 1);
+  const size_t wbytes = workspace_bytes(nodes, cap);
+  DeviceBuffers db;
+  void *d_idx, *d_lst, *d_st, *d_w;
+  if ((rc = db.alloc(&d_idx, sizeof(int32_t) * ((size_t)nodes + 1), "nidx_d"))) return rc;
+  if ((rc = db.alloc(&d_lst, sizeof(int32_t) * (size_t)edges, "nlist_d"))) return rc;
+  if ((rc = db.alloc(&d_st, sizeof(int32_t) * (size_t)nodes, "nstat_d"))) return rc;
+  if ((rc = db.alloc(&d_w, wbytes, "wl_d"))) return rc;
+  hipStream_t s = nullptr;
+  if ((rc = check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), ECLCC_ERR_DEVICE,
+                      "stream create failed")))
+    return rc;
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{s};
+  if ((rc = check_hip(hipMemcpyAsync(d_idx, nindex, sizeof(int32_t) * ((size_t)nodes + 1),
+                                     hipMemcpyHostToDevice, s),
+                      ECLCC_ERR_COPY, "copying to device failed")))
+    return rc;
+  if (edges > 0 &&
+      (rc = check_hip(hipMemcpyAsync(d_lst, nlist, sizeof(int32_t) * (size_t)edges,
+                                     hipMemcpyHostToDevice, s),
+                      ECLCC_ERR_COPY, "copying to device failed")))
+    return rc;
+  float kms[5], tot = 0.f;
+  rc = eclcc_run_device_profiled(nodes, edges, (const int32_t*)d_idx, (const int32_t*)d_lst,
+                                 (int32_t*)d_st, d_w, wbytes, params, s, kms, &tot);
+  if (rc) return rc;
+  if (seconds) *seconds = 0.001 * (double)tot;
+  if ((rc = check_hip(hipMemcpyAsync(nstat, d_st, sizeof(int32_t) * (size_t)nodes,
+                                     hipMemcpyDeviceToHost, s),
+                      ECLCC_ERR_COPY, "copying from device failed")))
+    return rc;
+  return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+}
+
+int eclcc_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
+                  int32_t* nstat) {
+  return eclcc_compute_timed(nodes, edges, nindex, nlist, nstat, nullptr, nullptr);
+}
+
+}  // extern "C"
+
+// ---- batches -------------------------------------------------------------------
+
+struct eclcc_batch {
+  int device = 0;
+  int32_t ngraphs = 0;
+  long long nslots = 0;
+  std::vector<GraphDesc> desc;   // host copy of the graph table
+  Tuning tuning{};
+  int32_t* d_nidx = nullptr;
+  int32_t* d_nlist = nullptr;
+  int32_t* d_nstat = nullptr;
+  GraphDesc* d_desc = nullptr;
+  int32_t* d_chunk = nullptr;
+  void* d_work = nullptr;
+  size_t work_bytes = 0;
+  Workspace w{};
+  // single-graph workspace for the one-graph-per-launch baseline
+  void* d_work1 = nullptr;
+  size_t work1_bytes = 0;
+  size_t device_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+
+  ~eclcc_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    for (void* p : {(void*)d_nidx, (void*)d_nlist, (void*)d_nstat, (void*)d_desc, (void*)d_chunk,
+                    d_work, d_work1})
+      if (p) (void)hipFree(p);
+  }
+
+  KParams params() const {
+    KParams p{};
+    p.nidx = d_nidx;
+    p.nlist = d_nlist;
+    p.nstat = d_nstat;
+    p.n = 0;
+    p.nslots = nslots;
+    p.desc = d_desc;
+    p.chunk_graph = d_chunk;
+    p.midbits = w.midbits;
+    p.hubs = w.hubs;
+    p.hubctr = w.hubctr;
+    p.flags = w.flags;
+    p.hub_cap = w.hub_cap;
+    p.thread_max = tuning.thread_max;
+    p.wave_max = tuning.wave_max;
+    p.hub_chunk = tuning.hub_chunk;
+    return p;
+  }
+};
+
+namespace {
+int batch_alloc(eclcc_batch* b, void** out, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 4));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_error(ECLCC_ERR_ALLOC, std::string("could not allocate ") + what + ": " +
+                                          hipGetErrorString(e));
+  }
+  b->device_bytes += bytes;
+  return ECLCC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edges,
+                       const int32_t* const* nindex, const int32_t* const* nlist,
+                       const eclcc_params* params, eclcc_batch** out) {
+  if (!out) return set_error(ECLCC_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (ngraphs < 1 || !nodes || !edges || !nindex || !nlist)
+    return set_error(ECLCC_ERR_ARG, "bad batch arguments");
+  std::unique_ptr<eclcc_batch> b(new (std::nothrow) eclcc_batch);
+  if (!b) return set_error(ECLCC_ERR_ALLOC, "host allocation failed");
+  int rc = resolve_tuning(params, &b->tuning);
+  if (rc) return rc;
+  (void)hipGetDevice(&b->device);
+  b->ngraphs = ngraphs;
+  b->desc.resize(ngraphs);
+  long long vb = 0, ib = 0, eb = 0, hub_cap = 0, max_n = 0, max_m = 0;
+  for (int32_t g = 0; g < ngraphs; g++) {
+    if ((rc = check_csr_args(nodes[g], edges[g], nindex[g], nlist[g]))) return rc;
+    GraphDesc& d = b->desc[g];
+    d.vbase = vb;
+    d.ibase = ib;
+    d.ebase = eb;
+    d.n = nodes[g];
+    d.m = edges[g];
+    vb += ((long long)nodes[g] + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+    ib += (long long)nodes[g] + 1;
+    eb += edges[g];
+    hub_cap += hub_capacity(edges[g], b->tuning.wave_max, 1);
+    max_n = std::max<long long>(max_n, nodes[g]);
+    max_m = std::max<long long>(max_m, edges[g]);
+  }
+  b->nslots = vb;
+  const long long nchunks = vb / kSlotAlign;
+  if ((rc = batch_alloc(b.get(), (void**)&b->d_nidx, sizeof(int32_t) * (size_t)ib, "batch nindex"))) return rc;
+  if ((rc = batch_alloc(b.get(), (void**)&b->d_nlist, sizeof(int32_t) * (size_t)eb, "batch nlist"))) return rc;
+  if ((rc = batch_alloc(b.get(), (void**)&b->d_nstat, sizeof(int32_t) * (size_t)vb, "batch labels"))) return rc;
+  if ((rc = batch_alloc(b.get(), (void**)&b->d_desc, sizeof(GraphDesc) * (size_t)ngraphs, "batch table"))) return rc;
+  if ((rc = batch_alloc(b.get(), (void**)&b->d_chunk, sizeof(int32_t) * (size_t)nchunks, "batch chunk map"))) return rc;
+  b->work_bytes = workspace_bytes(vb, hub_cap);
+  if ((rc = batch_alloc(b.get(), &b->d_work, b->work_bytes, "batch workspace"))) return rc;
+  if (!carve_workspace(b->d_work, b->work_bytes, vb, hub_cap, &b->w))
+    return set_error(ECLCC_ERR_ALLOC, "workspace carve failed");
+  b->work1_bytes = workspace_bytes(max_n, hub_capacity(max_m, b->tuning.wave_max, 1));
+  if ((rc = batch_alloc(b.get(), &b->d_work1, b->work1_bytes, "single-graph workspace"))) return rc;
+
+  std::vector<int32_t> chunk((size_t)nchunks);
+  for (int32_t g = 0; g < ngraphs; g++) {
+    const long long c0 = b->desc[g].vbase / kSlotAlign;
+    const long long c1 = (b->desc[g].vbase + b->desc[g].n + kSlotAlign - 1) / kSlotAlign;
+    for (long long c = c0; c < c1; c++) chunk[(size_t)c] = g;
+  }
+  const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+  if ((rc = check_hip(hipMemcpy(b->d_desc, b->desc.data(), sizeof(GraphDesc) * (size_t)ngraphs, h2d),
+                      ECLCC_ERR_COPY, "copying to device failed")))
+    return rc;
+  if ((rc = check_hip(hipMemcpy(b->d_chunk, chunk.data(), sizeof(int32_t) * (size_t)nchunks, h2d),
+                      ECLCC_ERR_COPY, "copying to device failed")))
+    return rc;
+  for (int32_t g = 0; g < ngraphs; g++) {
+    const GraphDesc& d = b->desc[g];
+    if ((rc = check_hip(hipMemcpy(b->d_nidx + d.ibase, nindex[g], sizeof(int32_t) * ((size_t)d.n + 1), h2d),
+                        ECLCC_ERR_COPY, "copying to device failed")))
+      return rc;
+    if (d.m > 0 &&
+        (rc = check_hip(hipMemcpy(b->d_nlist + d.ebase, nlist[g], sizeof(int32_t) * (size_t)d.m, h2d),
+                        ECLCC_ERR_COPY, "copying to device failed")))
+      return rc;
+  }
+  *out = b.release();
+  return ECLCC_OK;
+}
+
+int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph) {
+  if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
+  hipStream_t s = (hipStream_t)stream;
+  const KParams p = b->params();
+  const Shape shape{b->tuning.hub_blocks, b->tuning.block_threads};
+  if (!use_graph)
+    return check_hip(launch_pipeline(p, true, shape, s, nullptr), ECLCC_ERR_LAUNCH,
+                     "kernel launch failed");
+  if (!b->exec) {
+    int rc;
+    if (!b->capture_stream &&
+        (rc = check_hip(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking),
+                        ECLCC_ERR_DEVICE, "stream create failed")))
+      return rc;
+    if ((rc = check_hip(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal),
+                        ECLCC_ERR_LAUNCH, "graph capture failed")))
+      return rc;
+    hipError_t le = launch_pipeline(p, true, shape, b->capture_stream, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->capture_stream, &g);
+    if ((rc = check_hip(le, ECLCC_ERR_LAUNCH, "kernel launch failed during capture"))) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if ((rc = check_hip(ce, ECLCC_ERR_LAUNCH, "graph capture failed"))) return rc;
+    b->graph = g;
+    if ((rc = check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0),
+                        ECLCC_ERR_LAUNCH, "graph instantiate failed")))
+      return rc;
+  }
+  return check_hip(hipGraphLaunch(b->exec, s), ECLCC_ERR_LAUNCH, "graph launch failed");
+}
+
+int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
+  if (!b || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
+  const GraphDesc& d = b->desc[g];
+  eclcc_params prm;
+  eclcc_params_default(&prm);
+  prm.thread_max_deg = b->tuning.thread_max;
+  prm.wave_max_deg = b->tuning.wave_max;
+  prm.hub_chunk = b->tuning.hub_chunk;
+  prm.hub_blocks = b->tuning.hub_blocks;
+  prm.block_threads = b->tuning.block_threads;
+  return eclcc_run_device(d.n, d.m, b->d_nidx + d.ibase, b->d_nlist + d.ebase,
+                          b->d_nstat + d.vbase, b->d_work1, b->work1_bytes, &prm, stream);
+}
+
+int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat) {
+  if (!b || !nstat || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
+  const GraphDesc& d = b->desc[g];
+  return check_hip(hipMemcpy(nstat, b->d_nstat + d.vbase, sizeof(int32_t) * (size_t)d.n,
+                             hipMemcpyDeviceToHost),
+                   ECLCC_ERR_COPY, "copying from device failed");
+}
+
+int32_t* eclcc_batch_device_labels(eclcc_batch* b, int32_t g) {
+  if (!b || g < 0 || g >= b->ngraphs) return nullptr;
+  return b->d_nstat + b->desc[g].vbase;
+}
+
+size_t eclcc_batch_device_bytes(const eclcc_batch* b) { return b ? b->device_bytes : 0; }
+
+void eclcc_batch_free(eclcc_batch* b) { delete b; }
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+"""ctypes bindings of the two in-tree libraries.
+
+libeclcc.so    -- HIP kernels + C ABI (include/eclcc.h)
+libeclgraph.so -- host graph I/O and generators (include/eclgraph.h)
+
+Both are built in-tree by ``make -C ecl-suite_amd`` (``__graft_entry__.build()``).
+There is no fallback: if a library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_ROOT = Path(__file__).resolve().parent.parent      # .../ecl-suite_amd
+LIB_DIR = PKG_ROOT / "lib"
+BIN_DIR = PKG_ROOT / "bin"
+REPO_ROOT = PKG_ROOT.parent
+INCLUDE_DIR = REPO_ROOT / "include"
+
+i32 = C.c_int32
+i64 = C.c_int64
+u64 = C.c_uint64
+vp = C.c_void_p
+P_i32 = C.POINTER(C.c_int32)
+
+
+class EclccParams(C.Structure):
+    """Mirror of ``eclcc_params`` (include/eclcc.h)."""
+
+    _fields_ = [
+        ("thread_max_deg", i32),
+        ("wave_max_deg", i32),
+        ("hub_chunk", i32),
+        ("hub_blocks", i32),
+        ("block_threads", i32),
+        ("reserved", i32 * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class EclccServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_inflight", i32),
+        ("streams", i32),
+        ("max_batch", i32),
+        ("reserved", i32 * 5),
+    ]
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def _load(name: str) -> C.CDLL:
+    path = LIB_DIR / name
+    if not path.exists():
+        raise NativeLibraryMissing(
+            f"{path} is not built; run `make -C {PKG_ROOT}` (or __graft_entry__.build())")
+    return C.CDLL(str(path))
+
+
+_cc = None
+_graph = None
+
+
+def graph_lib() -> C.CDLL:
+    global _graph
+    if _graph is None:
+        L = _load("libeclgraph.so")
+        H = vp  # opaque eclgraph_csr*
+        L.eclgraph_last_error.restype = C.c_char_p
+        L.eclgraph_read.argtypes = [C.c_char_p, C.POINTER(H)]
+        L.eclgraph_write.argtypes = [H, C.c_char_p]
+        L.eclgraph_free.argtypes = [H]
+        L.eclgraph_free.restype = None
+        L.eclgraph_from_arrays.argtypes = [i32, i64, vp, vp, vp, C.POINTER(H)]
+        L.eclgraph_from_edges.argtypes = [i32, i64, vp, vp, C.c_int, C.POINTER(H)]
+        L.eclgraph_read_mtx.argtypes = [C.c_char_p, C.c_int, C.POINTER(H)]
+        L.eclgraph_nodes.argtypes = [H]
+        L.eclgraph_nodes.restype = i32
+        L.eclgraph_edges.argtypes = [H]
+        L.eclgraph_edges.restype = i64
+        for f in ("eclgraph_nindex", "eclgraph_nlist", "eclgraph_eweight"):
+            getattr(L, f).argtypes = [H]
+            getattr(L, f).restype = vp
+        L.eclgraph_gen_er.argtypes = [i32, i64, u64, C.POINTER(H)]
+        L.eclgraph_gen_rmat.argtypes = [C.c_int, i32, i64, C.c_double, C.c_double, C.c_double,
+                                        u64, C.c_int, C.POINTER(H)]
+        L.eclgraph_gen_grid3d.argtypes = [i32, i32, i32, C.c_double, u64, C.c_int, C.POINTER(H)]
+        L.eclgraph_rand64.argtypes = [u64, u64]
+        L.eclgraph_rand64.restype = u64
+        L.eclgraph_set_threads.argtypes = [C.c_int]
+        L.eclgraph_set_threads.restype = None
+        _graph = L
+    return _graph
+
+
+def cc_lib() -> C.CDLL:
+    global _cc
+    if _cc is None:
+        L = _load("libeclcc.so")
+        PP = C.POINTER(EclccParams)
+        L.eclcc_last_error.restype = C.c_char_p
+        L.eclcc_status_string.argtypes = [C.c_int]
+        L.eclcc_status_string.restype = C.c_char_p
+        L.eclcc_version.restype = C.c_char_p
+        L.eclcc_params_default.argtypes = [PP]
+        L.eclcc_params_default.restype = None
+        L.eclcc_compute.argtypes = [i32, i32, vp, vp, vp]
+        L.eclcc_compute_timed.argtypes = [i32, i32, vp, vp, vp, PP, C.POINTER(C.c_double)]
+        L.eclcc_workspace_bytes.argtypes = [i32, i64, PP]
+        L.eclcc_workspace_bytes.restype = C.c_size_t
+        L.eclcc_run_device.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp]
+        L.eclcc_run_device_profiled.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp,
+                                                C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.eclcc_run_init_only.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, vp]
+        L.eclcc_workspace_stats.argtypes = [vp, C.c_size_t, vp, C.POINTER(i64), C.POINTER(i64)]
+        L.eclcc_batch_create.argtypes = [i32, vp, vp, vp, vp, PP, C.POINTER(vp)]
+        L.eclcc_batch_run.argtypes = [vp, vp, C.c_int]
+        L.eclcc_batch_run_single.argtypes = [vp, i32, vp]
+        L.eclcc_batch_labels.argtypes = [vp, i32, vp]
+        L.eclcc_batch_device_labels.argtypes = [vp, i32]
+        L.eclcc_batch_device_labels.restype = vp
+        L.eclcc_batch_device_bytes.argtypes = [vp]
+        L.eclcc_batch_device_bytes.restype = C.c_size_t
+        L.eclcc_batch_free.argtypes = [vp]
+        L.eclcc_batch_free.restype = None
+        L.eclcc_server_create.argtypes = [C.POINTER(EclccServerConfig), PP, C.POINTER(vp)]
+        L.eclcc_server_submit.argtypes = [vp, i32, i32, vp, vp, vp]
+        L.eclcc_server_submit.restype = i64
+        L.eclcc_server_wait.argtypes = [vp, i64, i32, C.POINTER(C.c_double)]
+        L.eclcc_server_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64),
+                                         C.POINTER(C.c_size_t), C.POINTER(i32)]
+        L.eclcc_server_free.argtypes = [vp]
+        L.eclcc_server_free.restype = None
+        _cc = L
+    return _cc
+
+
+class EclccError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"eclcc status {status}: {message}")
+        self.status = status
+
+
+class EclgraphError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"eclgraph status {status}: {message}")
+        self.status = status
+
+
+def check_cc(rc: int) -> int:
+    if rc < 0:
+        L = cc_lib()
+        raise EclccError(rc, (L.eclcc_last_error() or b"").decode())
+    return rc
+
+
+def check_graph(rc: int) -> int:
+    if rc != 0:
+        L = graph_lib()
+        raise EclgraphError(rc, (L.eclgraph_last_error() or b"").decode())
+    return rc
+
+
+def header_functions(header: str) -> list[str]:
+    """Names of the functions an include/*.h header declares."""
+    import re
+
+    text = (INCLUDE_DIR / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = re.findall(r"\b((?:eclcc|eclgraph)_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+os.environ.setdefault("ECLGRAPH_THREADS", str(min(16, os.cpu_count() or 1)))

@@ -1,0 +1,76 @@
+"""Continuous-batching CC server (``eclcc_server_*`` in libeclcc.so).
+
+Requests are host CSR arrays; the native worker admits them by HBM budget,
+batches whatever has arrived into one launch sequence per stream, overlaps
+batches on several streams and copies the labels back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .graph import Graph
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes: int = 0, max_inflight: int = 0, streams: int = 0,
+                 max_batch: int = 0, params: Optional[N.EclccParams] = None):
+        cfg = N.EclccServerConfig()
+        cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+        cfg.max_inflight = int(max_inflight)
+        cfg.streams = int(streams)
+        cfg.max_batch = int(max_batch)
+        h = C.c_void_p()
+        N.check_cc(N.cc_lib().eclcc_server_create(C.byref(cfg),
+                                                  None if params is None else C.byref(params),
+                                                  C.byref(h)))
+        self._h = h
+        # request id -> (graph arrays kept alive, output array)
+        self._live: Dict[int, Tuple[np.ndarray, np.ndarray, np.ndarray]] = {}
+
+    def submit(self, g: Graph) -> int:
+        ni = np.ascontiguousarray(g.nindex, dtype=np.int32)
+        nl = np.ascontiguousarray(g.nlist, dtype=np.int32)
+        out = np.empty(g.nodes, dtype=np.int32)
+        rid = N.cc_lib().eclcc_server_submit(self._h, g.nodes, g.edges, ni.ctypes.data,
+                                             nl.ctypes.data if g.edges else None, out.ctypes.data)
+        N.check_cc(int(rid))
+        self._live[int(rid)] = (ni, nl, out)
+        return int(rid)
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> Tuple[np.ndarray, float]:
+        lat = C.c_double(0.0)
+        rc = N.cc_lib().eclcc_server_wait(self._h, int(rid), int(timeout_ms), C.byref(lat))
+        if rc == -9:  # timed out: the request stays live
+            N.check_cc(rc)
+        _, _, out = self._live.pop(int(rid))
+        N.check_cc(rc)
+        return out, lat.value
+
+    def stats(self) -> dict:
+        c, b, pk, mb = C.c_int64(0), C.c_int64(0), C.c_size_t(0), C.c_int32(0)
+        N.check_cc(N.cc_lib().eclcc_server_stats(self._h, C.byref(c), C.byref(b), C.byref(pk),
+                                                 C.byref(mb)))
+        return {"completed": c.value, "batches": b.value, "peak_device_bytes": pk.value,
+                "max_batch": mb.value}
+
+    def close(self) -> None:
+        if self._h is not None:
+            N.cc_lib().eclcc_server_free(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

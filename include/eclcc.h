@@ -1,0 +1,191 @@
+/*
+ * eclcc.h -- C ABI of the MI355X (gfx950) race-free ECL-CC connected
+ * components library, libeclcc.so.
+ *
+ * The reference's only in-process face for this path is
+ *     static void computeCC(const int nodes, const int edges,
+ *                           const int* nidx, const int* nlist, int* nstat)
+ * (src/racefree/egr-input/CC/ECL-CC-atomic.cu:275-325), file-local and
+ * called once from main (:363).  eclcc_compute() below is that function with
+ * a status return instead of exit(-1).  Everything else is added around it:
+ * device-resident runs for callers that keep the CSR in HBM, a batched form
+ * (many graphs, one launch sequence, capturable in a hipGraph) and a serving
+ * loop with admission by HBM budget.
+ *
+ * Data contract (same as the reference):
+ *   nindex[nodes+1], nlist[edges]: int32 CSR of a SYMMETRIC graph; IDs in
+ *   [0, nodes).  Self-loops and duplicate entries are harmless.
+ *   Output nstat[nodes]: nstat[v] = the minimum vertex ID in v's connected
+ *   component (the reference's final state; identical for every thread
+ *   interleaving).
+ *
+ * Conventions: every function returns ECLCC_OK (0) or a negative status and
+ * never exits the process; eclcc_last_error() gives the message (per host
+ * thread).  `stream` arguments are hipStream_t passed as void* (NULL = the
+ * null stream).  All device pointers belong to the current HIP device.
+ */
+#ifndef ECLCC_ABI_H
+#define ECLCC_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ECLCC_OK = 0,
+  ECLCC_ERR_ARG = -1,      /* invalid argument */
+  ECLCC_ERR_ALLOC = -2,    /* device or host allocation failed ("could not allocate ...") */
+  ECLCC_ERR_COPY = -3,     /* host<->device copy failed ("copying to/from device failed") */
+  ECLCC_ERR_LAUNCH = -4,   /* kernel launch or execution error (CheckCuda) */
+  ECLCC_ERR_DEVICE = -5,   /* no usable HIP device */
+  ECLCC_ERR_BUDGET = -6,   /* request larger than the HBM budget */
+  ECLCC_ERR_STATE = -7,    /* object in the wrong state / unknown id */
+  ECLCC_ERR_OVERFLOW = -8, /* internal worklist capacity exceeded */
+  ECLCC_ERR_TIMEOUT = -9   /* wait timed out */
+};
+
+const char* eclcc_last_error(void);
+const char* eclcc_status_string(int status);
+/* Library version string and the offload target it was built for. */
+const char* eclcc_version(void);
+
+/* ---- tuning parameters (degree thresholds and launch shapes) ---------- */
+typedef struct eclcc_params {
+  int32_t thread_max_deg; /* degree <= this: one lane per vertex (reference: 16, :120) */
+  int32_t wave_max_deg;   /* degree <= this: one 64-lane wavefront per vertex
+                             (reference: 352 = 11 x 32-lane warps, :122) */
+  int32_t hub_chunk;      /* edges per block task for degree > wave_max_deg */
+  int32_t hub_blocks;     /* blocks of the block-level kernel (0 = 8 per CU) */
+  int32_t block_threads;  /* threads per block of the block-level kernel (256..1024) */
+  int32_t reserved[3];
+} eclcc_params;
+
+/* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */
+void eclcc_params_default(eclcc_params* p);
+
+/* ---- plain drop-in ----------------------------------------------------- */
+
+/* Replaces computeCC (ECL-CC-atomic.cu:275-325): host CSR in, host labels
+   out.  Allocates, copies, runs init/compute1/compute2/compute3/flatten,
+   copies back and frees, like the reference.  edges is the CSR length. */
+int eclcc_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
+                  int32_t* nstat);
+
+/* Same, and *seconds = device time of the five kernels only (the window of
+   the reference's GPUTimer, ECL-CC-atomic.cu:305-312). params may be NULL. */
+int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
+                        int32_t* nstat, const eclcc_params* params, double* seconds);
+
+/* ---- device-resident (no allocation, no copies, asynchronous) ---------- */
+
+/* Device workspace the run needs (worklist bitmap, high-degree task list,
+   counters), in bytes. */
+size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params);
+
+/* Enqueues the five kernels on `stream` over device-resident CSR:
+   d_nstat[nodes] receives the labels; d_work is workspace of at least
+   eclcc_workspace_bytes() bytes.  Returns after launching (no sync). */
+int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                     const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
+                     const eclcc_params* params, void* stream);
+
+/* As eclcc_run_device, but records HIP events around each of the five
+   kernels, waits, and writes their durations (ms) to kernel_ms[5]
+   (init, compute1, compute2, compute3, flatten) and the whole window to
+   *total_ms. */
+int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                              const int32_t* d_nlist, int32_t* d_nstat, void* d_work,
+                              size_t work_bytes, const eclcc_params* params, void* stream,
+                              float* kernel_ms, float* total_ms);
+
+/* Test hook: runs only the init kernel (ECL-CC-atomic.cu:69-88) so its
+   closed form can be checked: d_nstat[v] = first neighbour < v in list
+   order, else v. */
+int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                        const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
+                        void* stream);
+
+/* Reads the worklist counters of the last run in d_work (synchronises
+   `stream`): number of high-degree vertices and of high-degree edge tasks. */
+int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, int64_t* n_hubs,
+                          int64_t* n_hub_tasks);
+
+/* ---- batched: many graphs, one launch sequence ------------------------- */
+
+/* A batch packs G graphs into one device allocation:
+     graph g's vertices occupy "slots" [slot0_g, slot0_g + n_g) of one label
+     array (slot0 padded to a multiple of 256), its nindex sits at
+     nindex_off_g in one packed int32 index array (n_g+1 local offsets), its
+     nlist at nlist_off_g (int64 offsets: the batch may exceed 2^31 entries
+     in total; each graph's own CSR stays int32).
+   The five kernels run once over all graphs.  Each graph's labels are the
+   same as eclcc_compute() on that graph alone. */
+typedef struct eclcc_batch eclcc_batch;
+
+/* Creates a batch on the current device and uploads the graphs
+   (host arrays; nindex[g] has nodes[g]+1 entries, nlist[g] edges[g]). */
+int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edges,
+                       const int32_t* const* nindex, const int32_t* const* nlist,
+                       const eclcc_params* params, eclcc_batch** out);
+
+/* Enqueues the batched run on `stream`; when use_graph != 0 the launch
+   sequence is captured once into a hipGraph and replayed afterwards. */
+int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph);
+
+/* Copies graph g's labels (nodes[g] ints) to host memory (synchronous). */
+int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat);
+
+/* Device pointer to graph g's labels (valid until eclcc_batch_free). */
+int32_t* eclcc_batch_device_labels(eclcc_batch* b, int32_t g);
+
+/* Total device bytes held by the batch. */
+size_t eclcc_batch_device_bytes(const eclcc_batch* b);
+
+void eclcc_batch_free(eclcc_batch* b);
+
+/* Runs graph g of a batch ALONE through the plain device path (five kernel
+   launches for that graph) on `stream`: the one-graph-per-launch baseline
+   the batched run is compared with, on the same resident data. */
+int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream);
+
+/* ---- served: continuous batching with admission by HBM budget ---------- */
+typedef struct eclcc_server eclcc_server;
+
+typedef struct eclcc_server_config {
+  size_t hbm_budget_bytes; /* device bytes the server may hold (0 = 80% of free HBM) */
+  int32_t max_inflight;    /* requests resident on the device at once (0 = 4096) */
+  int32_t streams;         /* concurrent batch streams (0 = 4) */
+  int32_t max_batch;       /* most requests fused into one launch (0 = 1024) */
+  int32_t reserved[5];
+} eclcc_server_config;
+
+/* Starts the serving thread on the current device. params may be NULL. */
+int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* params,
+                        eclcc_server** out);
+
+/* Submits one CC request.  The host arrays must stay valid and unchanged
+   until the request completes; labels_out[nodes] receives the result.
+   Returns a request id (>= 0) or a negative status (ECLCC_ERR_BUDGET if the
+   graph alone exceeds the budget). */
+int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
+                            const int32_t* nlist, int32_t* labels_out);
+
+/* Waits for request `id` (timeout_ms < 0: forever).  *latency_ms (optional)
+   = submit -> labels in host memory. */
+int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms);
+
+/* Server counters: completed requests, launched batches, peak device bytes,
+   largest batch. Any pointer may be NULL. */
+int eclcc_server_stats(eclcc_server* s, int64_t* completed, int64_t* batches,
+                       size_t* peak_bytes, int32_t* max_batch_seen);
+
+/* Drains outstanding requests and stops the server. */
+void eclcc_server_free(eclcc_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+"""GPU parity of the batched and served forms: every graph's labels equal
+the oracle's and the plain single-graph run's, alone and in the mix."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import announce
+from eclcc import compute, default_params, workloads
+from eclcc.batch import Batch
+from eclcc.server import Server
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_mix(count, seed=77, n_hi=20000):
+    return workloads.batch_mix(count, base_seed=seed, n_lo=1000, n_hi=n_hi)
+
+
+def test_batch_matches_oracle_and_single(cuda):
+    import torch
+
+    gs = small_mix(24)
+    announce(f"batch of {len(gs)} graphs, total m={sum(g.edges for g in gs)}")
+    expect = [oracle.cc_labels(g.nindex, g.nlist) for g in gs]
+    with Batch(gs) as b:
+        b.run(use_graph=False)
+        torch.cuda.synchronize()
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), g.name
+        # hipGraph: capture once, replay several times with host work between
+        for rep in range(4):
+            b.run(use_graph=True)
+            torch.cuda.synchronize()
+            got = b.all_labels()
+            for g, e, lab in zip(gs, expect, got):
+                assert np.array_equal(lab, e), f"replay {rep}: {g.name}"
+        # one graph per launch on the same resident data
+        for i in range(len(gs)):
+            b.run_single(i)
+        torch.cuda.synchronize()
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), g.name
+
+
+def test_batch_with_tuned_params(cuda):
+    import torch
+
+    gs = small_mix(10, seed=5, n_hi=60000)
+    announce("batch with non-default params")
+    p = default_params(thread_max_deg=2, wave_max_deg=32, hub_chunk=256, block_threads=512)
+    with Batch(gs, p) as b:
+        b.run(use_graph=True)
+        torch.cuda.synchronize()
+        for g, lab in zip(gs, b.all_labels()):
+            assert np.array_equal(lab, oracle.cc_labels(g.nindex, g.nlist)), g.name
+
+
+def test_server_alone_and_in_mix(cuda):
+    gs = small_mix(40, seed=91, n_hi=50000)
+    announce(f"server with {len(gs)} requests")
+    alone = [compute(g) for g in gs]
+    with Server(hbm_budget_bytes=256 << 20, streams=3, max_batch=16) as s:
+        # one at a time
+        for g, a in zip(gs[:5], alone[:5]):
+            lab, lat = s.wait(s.submit(g))
+            assert np.array_equal(lab, a) and lat > 0
+        # all at once: batched and overlapped
+        ids = [s.submit(g) for g in gs]
+        for g, a, rid in zip(gs, alone, ids):
+            lab, lat = s.wait(rid, timeout_ms=60000)
+            assert np.array_equal(lab, a), g.name
+            assert np.array_equal(lab, oracle.cc_labels(g.nindex, g.nlist)), g.name
+        st = s.stats()
+        assert st["completed"] == len(gs) + 5
+        assert st["max_batch"] > 1
+
+
+def test_server_rejects_request_over_budget(cuda):
+    g = workloads.rmat(14, 16, 3)
+    announce("server budget rejection")
+    from eclcc import EclccError
+
+    with Server(hbm_budget_bytes=1 << 20, streams=1) as s:
+        with pytest.raises(EclccError) as ei:
+            s.submit(g)
+        assert ei.value.status == -6

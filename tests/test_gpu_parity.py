@@ -1,0 +1,129 @@
+"""GPU parity: labels from libeclcc.so on an MI355X equal the oracle's,
+bit for bit (integer work: tolerance zero)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import announce
+import graphs as G
+from eclcc import DeviceGraph, compute, compute_timed, default_params, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def check(g, labels):
+    exp = oracle.cc_labels(g.nindex, g.nlist)
+    assert labels.dtype == np.int32 and labels.shape == (g.nodes,)
+    bad = np.nonzero(labels != exp)[0]
+    assert bad.size == 0, f"{g.name}: {bad.size} labels differ, first at {bad[:5]}"
+
+
+SMALL = [
+    lambda: G.empty(1),
+    lambda: G.empty(1000),
+    lambda: G.path(2),
+    lambda: G.path(5000),
+    lambda: G.star(3000, 0),
+    lambda: G.star(3000, 2999),
+    lambda: G.star(20000, 12345),
+    lambda: G.clique(90),
+    lambda: G.disjoint_cliques(40, 12, shuffle_seed=3),
+    lambda: G.two_stars_bridged(9000, 17, 30000, 40000),
+    lambda: workloads.er65536(1),
+    lambda: workloads.grid3d(32, 0.6, 1),
+    lambda: workloads.grid3d(24, 1.0, 2),
+    lambda: workloads.grid3d(24, 0.0, 2),
+    lambda: workloads.rmat(14, 16, 5),
+    lambda: workloads.rmat(16, 16, 6),
+]
+
+
+@pytest.mark.parametrize("make", SMALL, ids=[f"g{i}" for i in range(len(SMALL))])
+def test_plain_matches_oracle(cuda, make):
+    g = make()
+    announce(f"plain eclcc_compute {g.name} n={g.nodes} m={g.edges}")
+    labels, secs = compute_timed(g)
+    assert secs > 0
+    check(g, labels)
+
+
+def test_disjoint_cliques_closed_form(cuda):
+    g = G.disjoint_cliques(25, 7, shuffle_seed=11)
+    announce(f"closed form {g.name}")
+    assert np.array_equal(compute(g), G.expected_disjoint_cliques(25, 7, shuffle_seed=11))
+
+
+PARAMS = [
+    dict(thread_max_deg=0, wave_max_deg=1),          # nearly all work in compute3
+    dict(thread_max_deg=1, wave_max_deg=64),
+    dict(thread_max_deg=4, wave_max_deg=4096, hub_chunk=64),
+    dict(thread_max_deg=64, wave_max_deg=64, block_threads=512, hub_chunk=8192),
+    dict(thread_max_deg=16, wave_max_deg=704, block_threads=1024, hub_blocks=3),
+    dict(thread_max_deg=1 << 20, wave_max_deg=1 << 20),  # all at lane granularity
+]
+
+
+@pytest.mark.parametrize("kw", PARAMS, ids=[str(i) for i in range(len(PARAMS))])
+def test_thresholds_do_not_change_labels(cuda, kw):
+    g = workloads.rmat(15, 16, 9)
+    announce(f"params {kw} on {g.name}")
+    labels, _ = compute_timed(g, default_params(**kw))
+    check(g, labels)
+
+
+def test_init_kernel_closed_form(cuda):
+    # unsorted lists: init takes the FIRST smaller neighbour in list order
+    rng = np.random.default_rng(5)
+    g = workloads.rmat(14, 16, 2)
+    nl = g.nlist.copy()
+    for v in range(g.nodes):
+        b, e = g.nindex[v], g.nindex[v + 1]
+        rng.shuffle(nl[b:e])
+    g.nlist = nl
+    announce(f"init only {g.name}")
+    dg = DeviceGraph(g, "cuda:0")
+    dg.run_init_only()
+    assert np.array_equal(dg.labels(), oracle.init_labels(g.nindex, g.nlist))
+    dg.run()
+    check(g, dg.labels())
+
+
+def test_device_runs_are_repeatable(cuda):
+    g = workloads.rmat(16, 16, 12)
+    announce(f"repeat {g.name}")
+    dg = DeviceGraph(g, "cuda:0")
+    first = None
+    for _ in range(5):
+        dg.run()
+        lab = dg.labels()
+        if first is None:
+            first = lab
+            check(g, lab)
+        assert np.array_equal(lab, first)
+    kms, tot = dg.run_profiled()
+    assert len(kms) == 5 and tot > 0
+    assert np.array_equal(dg.labels(), first)
+
+
+def test_full_size_grid256(cuda):
+    g = workloads.grid256()
+    announce(f"full size {g.name} n={g.nodes} m={g.edges}")
+    dg = DeviceGraph(g, "cuda:0")
+    dg.run()
+    lab = dg.labels()
+    check(g, lab)
+    # size-independent properties
+    assert np.all(lab <= np.arange(g.nodes)) and np.all(lab[lab] == lab)
+
+
+def test_full_size_rmat24(cuda):
+    g = workloads.rmat24()
+    announce(f"full size {g.name} n={g.nodes} m={g.edges}")
+    dg = DeviceGraph(g, "cuda:0")
+    dg.run()
+    lab = dg.labels()
+    check(g, lab)
+    hubs, tasks = dg.hub_stats()
+    assert hubs > 0 and tasks >= hubs
