@@ -90,7 +90,7 @@ def plain(args) -> dict:
     for _ in range(args.warmup):
         dg.run(stream)
     torch.cuda.synchronize()
-    # per-kernel durations: HIP events between the five kernels on the launch stream
+    # per-kernel durations: HIP events between the kernels on the launch stream
     per = [[] for _ in KERNELS]
     tots = []
     for _ in range(max(3, min(args.steps, 10))):
@@ -132,7 +132,7 @@ def plain(args) -> dict:
         "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
         "roofline": {
             "bound": "hbm",
-            "kernel": "ECL-CC pass (init+compute1+compute2+compute3+flatten)",
+            "kernel": "ECL-CC pass (init, compute1, hubscan, hubtop, pick, mark, compute2, compute3, flatten)",
             "achieved": B / (pass_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

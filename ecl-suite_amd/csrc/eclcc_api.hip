@@ -59,6 +59,7 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   t.hub_chunk = p.hub_chunk;
   t.block_threads = p.block_threads;
   t.hub_blocks = p.hub_blocks;
+  t.filter = p.giant_filter ? 1 : 0;
   if (t.thread_max < 0 || t.wave_max < t.thread_max)
     return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
   if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
@@ -79,25 +80,56 @@ long long hub_capacity(long long edges, int wave_max, long long ngraphs) {
   return edges / ((long long)wave_max + 1) + ngraphs;
 }
 
-size_t workspace_bytes(long long nslots, long long hub_cap) {
-  size_t b = 256;                                              // counters, flags
-  b += align_up(sizeof(unsigned long long) * (size_t)((nslots + 63) / 64), 256);  // bitmap
+size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs) {
+  const size_t words = (size_t)((nslots + 63) / 64);
+  const size_t blocks = (words + kScanBlock - 1) / kScanBlock;
+  size_t b = 256;                                                   // counters, flags
+  b += 4 * align_up(sizeof(unsigned long long) * words, 256);       // mid, hub, info, member words
+  b += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
+  b += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
   b += align_up(sizeof(HubEntry) * (size_t)std::max<long long>(hub_cap, 1), 256);
   return b;
 }
 
-bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_cap, Workspace* w) {
-  if (!base || bytes < workspace_bytes(nslots, hub_cap)) return false;
+bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_cap,
+                     long long ngraphs, Workspace* w) {
+  if (!base || bytes < workspace_bytes(nslots, hub_cap, ngraphs)) return false;
   if ((reinterpret_cast<uintptr_t>(base) & 15) != 0) return false;
+  const size_t words = (size_t)((nslots + 63) / 64);
+  const size_t blocks = (words + kScanBlock - 1) / kScanBlock;
   char* p = static_cast<char*>(base);
   w->hubctr = reinterpret_cast<unsigned long long*>(p);
   w->flags = reinterpret_cast<unsigned int*>(p + 128);
   p += 256;
-  w->midbits = reinterpret_cast<unsigned long long*>(p);
-  p += align_up(sizeof(unsigned long long) * (size_t)((nslots + 63) / 64), 256);
+  unsigned long long** wordarrays[4] = {&w->midbits, &w->hubbits, &w->hubinfo, &w->gbits};
+  for (auto a : wordarrays) {
+    *a = reinterpret_cast<unsigned long long*>(p);
+    p += align_up(sizeof(unsigned long long) * words, 256);
+  }
+  w->blocktot = reinterpret_cast<unsigned long long*>(p);
+  p += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
+  w->giant = reinterpret_cast<int*>(p);
+  p += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
   w->hubs = reinterpret_cast<HubEntry*>(p);
   w->hub_cap = std::max<long long>(hub_cap, 1);
   return true;
+}
+
+void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
+  p->midbits = w.midbits;
+  p->hubbits = w.hubbits;
+  p->hubinfo = w.hubinfo;
+  p->gbits = w.gbits;
+  p->blocktot = w.blocktot;
+  p->giant = w.giant;
+  p->hubs = w.hubs;
+  p->hubctr = w.hubctr;
+  p->flags = w.flags;
+  p->hub_cap = w.hub_cap;
+  p->thread_max = t.thread_max;
+  p->wave_max = t.wave_max;
+  p->hub_chunk = t.hub_chunk;
+  p->filter = t.filter;
 }
 
 }  // namespace eclcc
@@ -120,17 +152,11 @@ KParams single_params(int32_t nodes, const int32_t* d_nindex, const int32_t* d_n
   p.nlist = d_nlist;
   p.nstat = d_nstat;
   p.n = nodes;
+  p.ngraphs = 1;
   p.nslots = nodes;
   p.desc = nullptr;
   p.chunk_graph = nullptr;
-  p.midbits = w.midbits;
-  p.hubs = w.hubs;
-  p.hubctr = w.hubctr;
-  p.flags = w.flags;
-  p.hub_cap = w.hub_cap;
-  p.thread_max = t.thread_max;
-  p.wave_max = t.wave_max;
-  p.hub_chunk = t.hub_chunk;
+  fill_params(&p, w, t);
   return p;
 }
 
@@ -187,7 +213,13 @@ const char* eclcc_status_string(int s) {
   }
 }
 
-const char* eclcc_version(void) { return "eclcc-mi355x 0.1 (gfx950, wave64)"; }
+const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64)"; }
+
+const char* eclcc_kernel_name(int k) {
+  static const char* names[kNumKernels] = {"init", "compute1", "hubscan", "hubtop", "pick",
+                                           "mark", "compute2", "compute3", "flatten"};
+  return (k >= 0 && k < kNumKernels) ? names[k] : "";
+}
 
 void eclcc_params_default(eclcc_params* p) {
   if (!p) return;
@@ -197,12 +229,13 @@ void eclcc_params_default(eclcc_params* p) {
   p->hub_chunk = 4096;
   p->hub_blocks = 0;
   p->block_threads = 256;
+  p->giant_filter = 1;
 }
 
 size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) {
   Tuning t;
   if (nodes < 1 || edges < 0 || resolve_tuning(params, &t) != ECLCC_OK) return 0;
-  return workspace_bytes(nodes, hub_capacity(edges, t.wave_max, 1));
+  return workspace_bytes(nodes, hub_capacity(edges, t.wave_max, 1), 1);
 }
 
 int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, const int32_t* d_nlist,
@@ -214,7 +247,7 @@ int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, cons
   Tuning t;
   if ((rc = resolve_tuning(params, &t))) return rc;
   Workspace w;
-  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), 1, &w))
     return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
   const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
   return check_hip(launch_pipeline(p, false, Shape{t.hub_blocks, t.block_threads},
@@ -232,11 +265,11 @@ int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nin
   Tuning t;
   if ((rc = resolve_tuning(params, &t))) return rc;
   Workspace w;
-  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), 1, &w))
     return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
   const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
-  hipEvent_t ev[6] = {};
-  for (int i = 0; i < 6; i++) {
+  hipEvent_t ev[kNumKernels + 1] = {};
+  for (int i = 0; i <= kNumKernels; i++) {
     if ((rc = check_hip(hipEventCreate(&ev[i]), ECLCC_ERR_DEVICE, "event create failed"))) {
       for (int j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
       return rc;
@@ -245,18 +278,18 @@ int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nin
   hipStream_t s = (hipStream_t)stream;
   rc = check_hip(launch_pipeline(p, false, Shape{t.hub_blocks, t.block_threads}, s, ev),
                  ECLCC_ERR_LAUNCH, "kernel launch failed");
-  if (!rc) rc = check_hip(hipEventSynchronize(ev[5]), ECLCC_ERR_LAUNCH, "kernel execution failed");
+  if (!rc) rc = check_hip(hipEventSynchronize(ev[kNumKernels]), ECLCC_ERR_LAUNCH, "kernel execution failed");
   if (!rc) {
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < kNumKernels; k++) {
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
       if (kernel_ms) kernel_ms[k] = ms;
     }
     float tot = 0.f;
-    (void)hipEventElapsedTime(&tot, ev[0], ev[5]);
+    (void)hipEventElapsedTime(&tot, ev[0], ev[kNumKernels]);
     if (total_ms) *total_ms = tot;
   }
-  for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i <= kNumKernels; i++) (void)hipEventDestroy(ev[i]);
   if (!rc) rc = overflow_check(w, s);
   return rc;
 }
@@ -270,7 +303,7 @@ int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
   Tuning t;
   if ((rc = resolve_tuning(nullptr, &t))) return rc;
   Workspace w;
-  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), &w))
+  if (!carve_workspace(d_work, work_bytes, nodes, hub_capacity(edges, t.wave_max, 1), 1, &w))
     return set_error(ECLCC_ERR_ARG, "workspace too small or misaligned");
   const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
   return check_hip(launch_init_only(p, (hipStream_t)stream), ECLCC_ERR_LAUNCH,
@@ -304,7 +337,7 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
     return set_error(ECLCC_ERR_DEVICE, "there is no HIP capable device");
   }
   const long long cap = hub_capacity(edges, t.wave_max, 1);
-  const size_t wbytes = workspace_bytes(nodes, cap);
+  const size_t wbytes = workspace_bytes(nodes, cap, 1);
   DeviceBuffers db;
   void *d_idx, *d_lst, *d_st, *d_w;
   if ((rc = db.alloc(&d_idx, sizeof(int32_t) * ((size_t)nodes + 1), "nidx_d"))) return rc;
@@ -328,7 +361,7 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
                                      hipMemcpyHostToDevice, s),
                       ECLCC_ERR_COPY, "copying to device failed")))
     return rc;
-  float kms[5], tot = 0.f;
+  float kms[kNumKernels], tot = 0.f;
   rc = eclcc_run_device_profiled(nodes, edges, (const int32_t*)d_idx, (const int32_t*)d_lst,
                                  (int32_t*)d_st, d_w, wbytes, params, s, kms, &tot);
   if (rc) return rc;
@@ -386,17 +419,11 @@ struct eclcc_batch {
     p.nlist = d_nlist;
     p.nstat = d_nstat;
     p.n = 0;
+    p.ngraphs = ngraphs;
     p.nslots = nslots;
     p.desc = d_desc;
     p.chunk_graph = d_chunk;
-    p.midbits = w.midbits;
-    p.hubs = w.hubs;
-    p.hubctr = w.hubctr;
-    p.flags = w.flags;
-    p.hub_cap = w.hub_cap;
-    p.thread_max = tuning.thread_max;
-    p.wave_max = tuning.wave_max;
-    p.hub_chunk = tuning.hub_chunk;
+    fill_params(&p, w, tuning);
     return p;
   }
 };
@@ -454,11 +481,11 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
   if ((rc = batch_alloc(b.get(), (void**)&b->d_nstat, sizeof(int32_t) * (size_t)vb, "batch labels"))) return rc;
   if ((rc = batch_alloc(b.get(), (void**)&b->d_desc, sizeof(GraphDesc) * (size_t)ngraphs, "batch table"))) return rc;
   if ((rc = batch_alloc(b.get(), (void**)&b->d_chunk, sizeof(int32_t) * (size_t)nchunks, "batch chunk map"))) return rc;
-  b->work_bytes = workspace_bytes(vb, hub_cap);
+  b->work_bytes = workspace_bytes(vb, hub_cap, ngraphs);
   if ((rc = batch_alloc(b.get(), &b->d_work, b->work_bytes, "batch workspace"))) return rc;
-  if (!carve_workspace(b->d_work, b->work_bytes, vb, hub_cap, &b->w))
+  if (!carve_workspace(b->d_work, b->work_bytes, vb, hub_cap, ngraphs, &b->w))
     return set_error(ECLCC_ERR_ALLOC, "workspace carve failed");
-  b->work1_bytes = workspace_bytes(max_n, hub_capacity(max_m, b->tuning.wave_max, 1));
+  b->work1_bytes = workspace_bytes(max_n, hub_capacity(max_m, b->tuning.wave_max, 1), 1);
   if ((rc = batch_alloc(b.get(), &b->d_work1, b->work1_bytes, "single-graph workspace"))) return rc;
 
   std::vector<int32_t> chunk((size_t)nchunks);
@@ -531,6 +558,7 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   prm.hub_chunk = b->tuning.hub_chunk;
   prm.hub_blocks = b->tuning.hub_blocks;
   prm.block_threads = b->tuning.block_threads;
+  prm.giant_filter = b->tuning.filter;
   return eclcc_run_device(d.n, d.m, b->d_nidx + d.ibase, b->d_nlist + d.ebase,
                           b->d_nstat + d.vbase, b->d_work1, b->work1_bytes, &prm, stream);
 }

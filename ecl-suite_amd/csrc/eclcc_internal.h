@@ -16,6 +16,11 @@ namespace eclcc {
 constexpr int kSlotAlign = 256;
 // Largest hub_chunk the LDS staging buffer of the block-level kernel holds.
 constexpr int kMaxHubChunk = 8192;
+// 64-slot words per block of the hub-list scan.
+constexpr int kScanBlock = 1024;
+// Vertices sampled per graph to find its largest component.
+constexpr int kPickSamples = 256;
+constexpr int kNumKernels = ECLCC_NUM_KERNELS;
 
 // One graph inside packed batch arrays.
 struct GraphDesc {
@@ -32,23 +37,31 @@ struct HubEntry {
   long long first_task;
 };
 
-// Everything the five kernels read; passed by value.
+// Everything the kernels read; passed by value.
 struct KParams {
   const int* nidx;   // single: nindex; batch: packed nindex
   const int* nlist;  // single: nlist; batch: packed nlist
   int* nstat;        // single: labels; batch: packed label slots
   int n;             // single: vertex count (unused for batches)
+  int ngraphs;       // 1 for a single graph
   long long nslots;  // slots covered by the vertex-parallel kernels
-  const GraphDesc* desc;    // batch: graph table
-  const int* chunk_graph;   // batch: graph of each 256-slot chunk
-  unsigned long long* midbits;  // one bit per slot: medium-degree vertex to process
-  HubEntry* hubs;               // high-degree vertices with their task offsets
-  unsigned long long* hubctr;   // low 32 bits: hubs pushed; high 32: edge tasks
-  unsigned int* flags;          // bit 0: hub list overflow
+  const GraphDesc* desc;   // batch: graph table
+  const int* chunk_graph;  // batch: graph of each 256-slot chunk
+  // one 64-bit word per 64 slots
+  unsigned long long* midbits;   // medium-degree vertices (compute1 -> compute2)
+  unsigned long long* hubbits;   // high-degree vertices (compute1 -> mark)
+  unsigned long long* hubinfo;   // hubs | tasks << 32 of the word, then its prefix in its block
+  unsigned long long* gbits;     // members of the sampled largest component (mark -> compute2/3)
+  unsigned long long* blocktot;  // per kScanBlock words: totals, then exclusive prefix
+  int* giant;                    // per graph: sampled root of the largest component, or -1
+  HubEntry* hubs;                // high-degree vertices with their first task number
+  unsigned long long* hubctr;    // low 32 bits: hubs; high 32: edge tasks
+  unsigned int* flags;           // bit 0: hub list overflow
   long long hub_cap;
   int thread_max;
   int wave_max;
   int hub_chunk;
+  int filter;  // 1: skip edges with both ends in the sampled largest component
 };
 
 // Workspace carved out of one device buffer.
@@ -56,6 +69,11 @@ struct Workspace {
   unsigned long long* hubctr;
   unsigned int* flags;
   unsigned long long* midbits;
+  unsigned long long* hubbits;
+  unsigned long long* hubinfo;
+  unsigned long long* gbits;
+  unsigned long long* blocktot;
+  int* giant;
   HubEntry* hubs;
   long long hub_cap;
 };
@@ -67,15 +85,17 @@ struct Shape {
 
 // Resolved, validated parameters.
 struct Tuning {
-  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads;
+  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter;
 };
 
-size_t workspace_bytes(long long nslots, long long hub_cap);
-bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_cap, Workspace* w);
+size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs);
+bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_cap,
+                     long long ngraphs, Workspace* w);
 long long hub_capacity(long long edges, int wave_max, long long ngraphs);
+void fill_params(KParams* p, const Workspace& w, const Tuning& t);
 
-// Enqueues init, compute1, compute2, compute3, flatten on `s`.  If ev is not
-// null, ev[0..5] are recorded before, between and after the kernels.
+// Enqueues the kernel sequence on `s`.  If ev is not null, ev[0..kNumKernels]
+// are recorded before, between and after the kernels.
 hipError_t launch_pipeline(const KParams& p, bool batch, const Shape& shape, hipStream_t s,
                            hipEvent_t* ev);
 hipError_t launch_init_only(const KParams& p, hipStream_t s);

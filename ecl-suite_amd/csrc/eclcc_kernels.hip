@@ -1,4 +1,4 @@
-// eclcc_kernels.hip -- the five race-free ECL-CC kernels for gfx950.
+// eclcc_kernels.hip -- race-free ECL-CC kernels for gfx950 (MI355X).
 //
 // Reference algorithm: src/racefree/egr-input/CC/ECL-CC-atomic.cu
 //   init :69-88, representative :92-104, compute1 :108-157,
@@ -9,19 +9,26 @@
 // (relaxed, agent scope).  The only plain label store is in init, where no
 // other thread reads labels.
 //
-// MI355X-specific structure (64-lane wavefronts):
-//   * compute1 processes degree <= thread_max at one lane per vertex and, in
-//     the same pass, records medium-degree vertices as one 64-bit ballot word
-//     per wavefront (a bitmap worklist: no atomics, no ordering), and pushes
-//     high-degree vertices with one wave-aggregated 64-bit atomic that
-//     reserves both list slots and edge-task numbers.
-//   * compute2 gives each medium-degree vertex one whole wavefront.
-//   * compute3 splits every high-degree adjacency list into hub_chunk-edge
-//     tasks, so one hub is spread over many blocks; each task is read with
-//     coalesced loads, filtered (only neighbours < v are hooked) and
-//     compacted into LDS, then hooked with all lanes busy.
-//   * Batches: the same kernels run over packed graphs; each 64-lane
-//     wavefront resolves its graph once from a chunk->graph table.
+// Kernel sequence (one launch each, no host synchronisation in between):
+//   init      labels := first smaller neighbour (reference init)
+//   compute1  degree <= thread_max: one lane per vertex; records medium and
+//             high-degree vertices as 64-bit ballot words (no atomics)
+//   hubscan   exclusive prefix of the high-degree counts per 1024-word block
+//   hubtop    prefix over the blocks; total hubs and edge tasks
+//   pick      per graph: the most frequent root among 256 sampled vertices
+//   mark      bitmap of the vertices already joined to that root; writes the
+//             high-degree task list in slot order
+//   compute2  medium degree: wavefront granularity
+//   compute3  high degree: block granularity, edge tasks staged through LDS
+//   flatten   labels := root (reference flatten)
+//
+// Why the bitmap (MI355X measurements): random 4-byte gathers from the
+// 64 MB label array of a 2^24-vertex graph run at ~58 G/s (memory side),
+// from a 2 MB table that stays in L2 at ~265 G/s.  After compute1 most
+// medium- and high-degree edges of a power-law graph join two vertices that
+// are already in the largest tree; one bit per vertex (2 MB for 2^24) tells
+// that without touching the labels.  Skipping an edge whose ends are both
+// connected to the sampled root is exact: they are in one component already.
 
 #include <climits>
 
@@ -44,6 +51,9 @@ __device__ __forceinline__ int lab_cas(int* p, int expect, int desired) {
   return expect;
 }
 
+// adjacency entries are streamed once: keep them from evicting the bitmap
+__device__ __forceinline__ int ld_stream(const int* p) { return __builtin_nontemporal_load(p); }
+
 // ---- wavefront helpers ------------------------------------------------------
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
@@ -56,6 +66,23 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// inclusive prefix sum across the wavefront
+__device__ __forceinline__ unsigned wave_incl_sum(unsigned x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned t = __shfl_up(x, d);
+    if (lane >= d) x += t;
+  }
+  return x;
+}
+
+// bit `slot` of a per-slot bitmap
+__device__ __forceinline__ unsigned slot_bit(const unsigned long long* bits, long long slot) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(bits);
+  return (w[slot >> 5] >> (slot & 31)) & 1u;
+}
+
 // ---- graph resolution -------------------------------------------------------
 
 struct GraphRef {
@@ -63,18 +90,24 @@ struct GraphRef {
   const int* nlist;
   int* nstat;
   int n;
+  int g;
   long long slot0;
 };
 
 template <bool BATCH>
-__device__ __forceinline__ GraphRef resolve(const KParams& p, long long slot) {
+__device__ __forceinline__ GraphRef graph_ref(const KParams& p, int g) {
   if constexpr (!BATCH) {
-    return GraphRef{p.nidx, p.nlist, p.nstat, p.n, 0};
+    return GraphRef{p.nidx, p.nlist, p.nstat, p.n, 0, 0};
   } else {
-    const int g = p.chunk_graph[slot >> 8];
     const GraphDesc d = p.desc[g];
-    return GraphRef{p.nidx + d.ibase, p.nlist + d.ebase, p.nstat + d.vbase, d.n, d.vbase};
+    return GraphRef{p.nidx + d.ibase, p.nlist + d.ebase, p.nstat + d.vbase, d.n, g, d.vbase};
   }
+}
+
+template <bool BATCH>
+__device__ __forceinline__ GraphRef resolve(const KParams& p, long long slot) {
+  if constexpr (!BATCH) return graph_ref<false>(p, 0);
+  else return graph_ref<true>(p, p.chunk_graph[slot >> 8]);
 }
 
 // first slot of this thread's wavefront, made wave-uniform
@@ -105,6 +138,22 @@ __device__ __forceinline__ int find_root(int idx, int* nstat) {
   return find_from(idx, lab_ld(&nstat[idx]), nstat);
 }
 
+// Like find_from, but stops as soon as the walk reaches `stop` (a known
+// ancestor of the caller's vertex, usually the root of the largest tree):
+// the two are then joined, and the label of `stop` -- read by every thread
+// otherwise -- is not touched.
+__device__ __forceinline__ int find_until(int idx, int curr, int stop, int* nstat) {
+  if (curr != idx) {
+    int prev = idx, next;
+    while (curr != stop && curr > (next = lab_ld(&nstat[curr]))) {
+      lab_st(&nstat[prev], next);
+      prev = curr;
+      curr = next;
+    }
+  }
+  return curr;
+}
+
 // Min-linking of two root estimates (ECL-CC-atomic.cu:134-151): the larger
 // root is swung to the smaller by CAS; a failed CAS returns the value that
 // is there now, which is strictly smaller, and the walk continues from it.
@@ -128,11 +177,26 @@ __device__ __forceinline__ int hook(int vstat, int ostat, int* nstat) {
 }
 
 // Hooks the edge v-u (caller checked v > u); s = label[u] already loaded.
-// When u's parent already is v's root estimate the two are joined and
-// nothing is written.
+// When u's path reaches v's root estimate the two are joined and nothing is
+// written.
 __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat) {
   if (s == vstat) return vstat;
-  return hook(vstat, find_from(u, s, nstat), nstat);
+  const int r = find_until(u, s, vstat, nstat);
+  if (r == vstat) return vstat;
+  return hook(vstat, r, nstat);
+}
+
+// Hooks the neighbours u[0..K) of v that are smaller than v; all K label
+// loads are in flight before the first one is used.
+template <int K>
+__device__ __forceinline__ int hook_batch(int v, int vstat, const int (&u)[K], int* ns) {
+  int s[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) s[k] = v > u[k] ? lab_ld(&ns[u[k]]) : 0;
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (v > u[k]) vstat = hook_edge(vstat, u[k], s[k], ns);
+  return vstat;
 }
 
 // ---- init (ECL-CC-atomic.cu:69-88) --------------------------------------------
@@ -189,40 +253,11 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
 }
 
 // ---- compute1 (ECL-CC-atomic.cu:108-157) --------------------------------------
-
-// Wave-aggregated push of high-degree vertices: one 64-bit atomic reserves
-// list slots (low half) and edge-task numbers (high half) for the whole wave.
-__device__ __forceinline__ void push_hubs(const KParams& p, bool hub, long long slot, int deg,
-                                          unsigned long long mask) {
-  const int lane = lane_id();
-  const unsigned tasks = hub ? (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk) : 0u;
-  unsigned incl = tasks;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const unsigned t = __shfl_up(incl, d);
-    if (lane >= d) incl += t;
-  }
-  const unsigned total = __shfl(incl, 63);
-  unsigned lo = 0, hi = 0;
-  if (lane == 0) {
-    const unsigned long long add =
-        (unsigned long long)__popcll(mask) | ((unsigned long long)total << 32);
-    const unsigned long long old =
-        __hip_atomic_fetch_add(p.hubctr, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    lo = (unsigned)old;
-    hi = (unsigned)(old >> 32);
-  }
-  lo = __shfl(lo, 0);
-  hi = __shfl(hi, 0);
-  if (hub) {
-    const long long idx = (long long)lo + lanes_below(mask);
-    if (idx < p.hub_cap) {
-      p.hubs[idx] = HubEntry{slot, (long long)hi + (long long)(incl - tasks)};
-    } else {
-      __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
+// Degree binning and lane-granularity hooking.  Medium and high-degree
+// vertices are not pushed to a shared worklist (one contended device-scope
+// atomic per push costs ~10 ns, serialised, on MI355X): each wavefront stores
+// its two ballot words, and for the high-degree ones the count and the
+// number of edge tasks.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
@@ -233,89 +268,267 @@ __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
-  bool active = false;
-  int beg = 0, end = 0;
-  if (live) {
-    active = lab_ld(&G.nstat[v]) != v;  // roots after init have no smaller neighbour
-    if (active) {
-      beg = G.nidx[v];
-      end = G.nidx[v + 1];
-    }
+  int s0 = v, beg = 0, end = 0;
+  if (live) {  // independent loads, in flight together
+    s0 = lab_ld(&G.nstat[v]);
+    beg = G.nidx[v];
+    end = G.nidx[v + 1];
   }
-  const int deg = end - beg;
-  const bool mid = active && deg > p.thread_max && deg <= p.wave_max;
-  const bool hub = active && deg > p.wave_max;
+  const bool active = live && s0 != v;  // roots after init have no smaller neighbour
+  const int deg = active ? end - beg : 0;
+  const bool mid = deg > p.thread_max && deg <= p.wave_max;
+  const bool hub = deg > p.wave_max;
   const unsigned long long midmask = __ballot(mid);
-  if (lane == 0) p.midbits[ws >> 6] = midmask;
   const unsigned long long hubmask = __ballot(hub);
-  if (hubmask) push_hubs(p, hub, slot, deg, hubmask);
+  unsigned long long info = 0;
+  if (hubmask) {
+    unsigned tasks = hub ? (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk) : 0u;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
+    info = (unsigned long long)__popcll(hubmask) | ((unsigned long long)tasks << 32);
+  }
+  if (lane == 0) {
+    const long long w = ws >> 6;
+    p.midbits[w] = midmask;
+    p.hubbits[w] = hubmask;
+    p.hubinfo[w] = info;
+  }
 
   if (active && deg <= p.thread_max) {
     int* const ns = G.nstat;
     const int* const nl = G.nlist;
-    int vstat = find_root(v, ns);
-    // four neighbours per step: their label loads are issued together
-    for (int i = beg; i < end; i += 4) {
-      const int u0 = nl[i];
-      const int u1 = i + 1 < end ? nl[i + 1] : INT_MAX;
-      const int u2 = i + 2 < end ? nl[i + 2] : INT_MAX;
-      const int u3 = i + 3 < end ? nl[i + 3] : INT_MAX;
-      const int s0 = v > u0 ? lab_ld(&ns[u0]) : 0;
-      const int s1 = v > u1 ? lab_ld(&ns[u1]) : 0;
-      const int s2 = v > u2 ? lab_ld(&ns[u2]) : 0;
-      const int s3 = v > u3 ? lab_ld(&ns[u3]) : 0;
-      if (v > u0) vstat = hook_edge(vstat, u0, s0, ns);
-      if (v > u1) vstat = hook_edge(vstat, u1, s1, ns);
-      if (v > u2) vstat = hook_edge(vstat, u2, s2, ns);
-      if (v > u3) vstat = hook_edge(vstat, u3, s3, ns);
+    // the first neighbours are fetched while v's root is being found
+    int u[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) u[k] = beg + k < end ? ld_stream(&nl[beg + k]) : INT_MAX;
+    int vstat = find_from(v, s0, ns);
+    vstat = hook_batch<8>(v, vstat, u, ns);
+    for (int i = beg + 8; i < end; i += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : INT_MAX;
+      vstat = hook_batch<8>(v, vstat, u, ns);
     }
   }
 }
 
+// ---- high-degree task list: block prefix, grand prefix, emission -----------
+
+// Block-wide exclusive scan of packed (count | tasks << 32) values, one per
+// thread of a 1024-thread block; returns the exclusive prefix and sets *tot.
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long x,
+                                                              unsigned long long* tot) {
+  __shared__ unsigned s_lo[16], s_hi[16];
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  const unsigned lo = wave_incl_sum((unsigned)x);
+  const unsigned hi = wave_incl_sum((unsigned)(x >> 32));
+  if (lane == 63) {
+    s_lo[wv] = lo;
+    s_hi[wv] = hi;
+  }
+  __syncthreads();
+  unsigned plo = 0, phi = 0, tlo = 0, thi = 0;
+  for (int k = 0; k < 16; k++) {
+    if (k < wv) {
+      plo += s_lo[k];
+      phi += s_hi[k];
+    }
+    tlo += s_lo[k];
+    thi += s_hi[k];
+  }
+  __syncthreads();
+  *tot = (unsigned long long)tlo | ((unsigned long long)thi << 32);
+  const unsigned elo = plo + lo - (unsigned)x;
+  const unsigned ehi = phi + hi - (unsigned)(x >> 32);
+  return (unsigned long long)elo | ((unsigned long long)ehi << 32);
+}
+
+__global__ __launch_bounds__(1024) void k_hub_scan(KParams p) {
+  const long long words = (p.nslots + 63) / 64;
+  const long long w = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+  const unsigned long long x = w < words ? p.hubinfo[w] : 0ull;
+  unsigned long long tot;
+  const unsigned long long ex = block_excl_scan(x, &tot);
+  if (w < words) p.hubinfo[w] = ex;
+  if (threadIdx.x == 0) p.blocktot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_hub_top(KParams p) {
+  const long long words = (p.nslots + 63) / 64;
+  const long long nb = (words + kScanBlock - 1) / kScanBlock;
+  unsigned long long carry = 0;
+  for (long long b0 = 0; b0 < nb; b0 += 1024) {
+    const long long b = b0 + threadIdx.x;
+    const unsigned long long x = b < nb ? p.blocktot[b] : 0ull;
+    unsigned long long tot;
+    const unsigned long long ex = block_excl_scan(x, &tot);
+    if (b < nb) p.blocktot[b] = ex + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *p.hubctr = carry;
+}
+
+// ---- pick: per graph, the most frequent root among sampled vertices -------
+
+template <bool BATCH>
+__global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
+  __shared__ int s_root[kPickSamples];
+  __shared__ unsigned long long s_best[kPickSamples / 64];
+  const int g = blockIdx.x;
+  const GraphRef G = graph_ref<BATCH>(p, g);
+  const int t = threadIdx.x;
+  int r = -1;
+  if (p.filter && G.n >= 2) {
+    const int nsamp = G.n < kPickSamples ? G.n : kPickSamples;
+    if (t < nsamp) {
+      const int v = (int)(((long long)t * G.n) / nsamp);
+      r = find_root(v, G.nstat);
+    }
+  }
+  s_root[t] = r;
+  __syncthreads();
+  unsigned c = 0;
+  if (r >= 0)
+    for (int k = 0; k < kPickSamples; k++) c += s_root[k] == r;
+  // largest count, ties to the smaller root
+  unsigned long long key = r >= 0 ? ((unsigned long long)c << 32) | (unsigned)(INT_MAX - r) : 0ull;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(key, d);
+    key = o > key ? o : key;
+  }
+  if (lane_id() == 0) s_best[t >> 6] = key;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long best = 0;
+    for (int k = 0; k < kPickSamples / 64; k++) best = s_best[k] > best ? s_best[k] : best;
+    const unsigned cnt = (unsigned)(best >> 32);
+    p.giant[g] = cnt >= 2 ? INT_MAX - (int)(unsigned)best : -1;
+  }
+}
+
+// ---- mark: membership bitmap and the high-degree task list -----------------
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_mark(KParams p) {
+  const long long ws = wave_slot();
+  if (ws >= p.nslots) return;
+  const GraphRef G = resolve<BATCH>(p, ws);
+  const int lane = lane_id();
+  const long long w = ws >> 6;
+  const long long slot = ws + lane;
+  const int v = (int)(slot - G.slot0);
+  const bool live = slot < p.nslots && v < G.n;
+  // high-degree vertices of this word, in slot order
+  const unsigned long long hb = p.hubbits[w];
+  if (hb) {
+    const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
+    const bool hub = (hb >> lane) & 1ull;
+    unsigned tasks = 0;
+    if (hub) tasks = (unsigned)((G.nidx[v + 1] - G.nidx[v] + p.hub_chunk - 1) / p.hub_chunk);
+    const unsigned excl = wave_incl_sum(tasks) - tasks;
+    if (hub) {
+      const long long idx = (long long)(unsigned)base + lanes_below(hb);
+      if (idx < p.hub_cap) {
+        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl};
+      } else {
+        __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  // vertices already joined to the graph's sampled root
+  const int R = p.filter ? p.giant[G.g] : -1;
+  bool member = false;
+  if (R >= 0 && live) {
+    const int s = lab_ld(&G.nstat[v]);
+    member = v == R || find_until(v, s, R, G.nstat) == R;
+  }
+  const unsigned long long gw = __ballot(member);
+  if (lane == 0) p.gbits[w] = gw;
+}
+
 // ---- compute2 (ECL-CC-atomic.cu:161-199) --------------------------------------
-// One wavefront per medium-degree vertex.  Wavefront w owns bitmap word w,
-// i.e. the 64 slots compute1's wavefront w looked at.
+// Wavefront w owns bitmap word w, i.e. the 64 slots compute1's wavefront w
+// looked at, and processes all medium-degree vertices among them together:
+// their adjacency lists are laid end to end and dealt out evenly over the
+// 64 lanes, four entries per lane per step, so a wavefront never idles on a
+// short list.  An entry's vertex is found by a binary search over the lanes'
+// list offsets (cross-lane shuffles); its root estimate comes from the lane
+// that owns the vertex, which found it once for all of its entries.  Entries
+// whose two ends are both marked members of the largest tree are skipped
+// after one bitmap lookup.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  unsigned long long bits = p.midbits[ws >> 6];
+  const unsigned long long bits = p.midbits[ws >> 6];
   if (!bits) return;
   const GraphRef G = resolve<BATCH>(p, ws);
-  const int lane = lane_id();
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
-  while (bits) {
-    const int b = __ffsll((long long)bits) - 1;
-    bits &= bits - 1;
-    const int v = (int)(ws + b - G.slot0);
-    const int beg = G.nidx[v];
-    const int end = G.nidx[v + 1];
-    int vstat = find_root(v, ns);
-    int i = beg + lane;
-    for (; i + 64 < end; i += 128) {
-      const int u0 = nl[i];
-      const int u1 = nl[i + 64];
-      const int s0 = v > u0 ? lab_ld(&ns[u0]) : 0;
-      const int s1 = v > u1 ? lab_ld(&ns[u1]) : 0;
-      if (v > u0) vstat = hook_edge(vstat, u0, s0, ns);
-      if (v > u1) vstat = hook_edge(vstat, u1, s1, ns);
+  const int lane = lane_id();
+  const bool mine = (bits >> lane) & 1ull;
+  const int v = (int)(ws + lane - G.slot0);
+  const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
+  int beg = 0, deg = 0, root = 0;
+  if (mine) {
+    root = lab_ld(&ns[v]);
+    beg = G.nidx[v];
+    deg = G.nidx[v + 1] - beg;
+  }
+  if (mine) root = find_from(v, root, ns);
+  const int incl = (int)wave_incl_sum((unsigned)deg);
+  const int off = incl - deg;
+  const int total = __shfl(incl, 63);
+  const int vbase = (int)(ws - G.slot0);
+  for (int e0 = 0; e0 < total; e0 += 256) {
+    int u[4], src[4], vr[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = e0 + k * 64 + lane;
+      int sl = 0;  // last lane whose list starts at or before entry e
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(off, sl + step) <= e) sl += step;
+      src[k] = sl;
+      const int lb = __shfl(beg, sl);
+      const int lo = __shfl(off, sl);
+      vr[k] = __shfl(root, sl);
+      u[k] = e < total ? ld_stream(&nl[lb + (e - lo)]) : INT_MAX;
     }
-    if (i < end) {
-      const int u = nl[i];
-      if (v > u) vstat = hook_edge(vstat, u, lab_ld(&ns[u]), ns);
+    bool need[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) need[k] = vbase + src[k] > u[k];
+    if (gw) {
+      unsigned in[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        in[k] = (need[k] && ((gw >> src[k]) & 1ull)) ? slot_bit(p.gbits, G.slot0 + u[k]) : 0u;
+#pragma unroll
+      for (int k = 0; k < 4; k++) need[k] = need[k] && !in[k];
     }
+    int s[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) s[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (need[k]) (void)hook_edge(vr[k], u[k], s[k], ns);
   }
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------
 // Blocks stride over the edge tasks of all high-degree vertices.  A task is
-// hub_chunk consecutive entries of one adjacency list: read coalesced,
-// neighbours < v compacted into LDS, then hooked by every thread.
+// hub_chunk consecutive entries of one adjacency list: every thread issues
+// its share of the coalesced loads at once, the entries that still need a
+// hook (< v, and not both ends in the largest tree) are compacted into LDS
+// with wave ballots, and the compacted entries are hooked with all threads
+// busy and up to 16 label loads in flight per thread.  Adjacency lists are
+// sorted, so the bitmap lookups of one task fall on few cache lines.
 
 template <bool BATCH, int BT>
 __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
+  constexpr int R = 16;
   __shared__ int stage[kMaxHubChunk];
   __shared__ int s_cnt;
   const unsigned long long ctr = *p.hubctr;
@@ -335,22 +548,33 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
     const HubEntry h = p.hubs[lo];
     const GraphRef G = resolve<BATCH>(p, h.slot);
     const int v = (int)(h.slot - G.slot0);
+    const bool vin = p.filter && slot_bit(p.gbits, h.slot);
     const long long cb = (long long)G.nidx[v] + (t - h.first_task) * (long long)p.hub_chunk;
     const long long ce_full = G.nidx[v + 1];
     const int cbeg = (int)cb;
     const int cend = (int)(cb + p.hub_chunk < ce_full ? cb + p.hub_chunk : ce_full);
     if (tid == 0) s_cnt = 0;
     __syncthreads();
-    for (int base = cbeg; base < cend; base += BT) {
-      const int i = base + tid;
-      const int u = i < cend ? G.nlist[i] : INT_MAX;
-      const bool keep = u < v;
-      const unsigned long long mask = __ballot(keep);
-      if (mask) {
-        int off = 0;
-        if (lane == 0) off = atomicAdd(&s_cnt, __popcll(mask));
-        off = __shfl(off, 0);
-        if (keep) stage[off + lanes_below(mask)] = u;
+    for (int base = cbeg; base < cend; base += R * BT) {
+      int u[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = base + r * BT + tid;
+        u[r] = i < cend ? ld_stream(&G.nlist[i]) : INT_MAX;
+      }
+      unsigned in[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) in[r] = (vin && u[r] < v) ? slot_bit(p.gbits, G.slot0 + u[r]) : 0u;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const bool keep = u[r] < v && !in[r];
+        const unsigned long long mask = __ballot(keep);
+        if (mask) {
+          int off = 0;
+          if (lane == 0) off = atomicAdd(&s_cnt, __popcll(mask));
+          off = __shfl(off, 0);
+          if (keep) stage[off + lanes_below(mask)] = u[r];
+        }
       }
     }
     __syncthreads();
@@ -358,18 +582,18 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
     if (cnt > 0) {
       int* const ns = G.nstat;
       int vstat = find_root(v, ns);
-      int j = tid;
-      for (; j + BT < cnt; j += 2 * BT) {
-        const int u0 = stage[j];
-        const int u1 = stage[j + BT];
-        const int s0 = lab_ld(&ns[u0]);
-        const int s1 = lab_ld(&ns[u1]);
-        vstat = hook_edge(vstat, u0, s0, ns);
-        vstat = hook_edge(vstat, u1, s1, ns);
-      }
-      if (j < cnt) {
-        const int u = stage[j];
-        vstat = hook_edge(vstat, u, lab_ld(&ns[u]), ns);
+      for (int j0 = 0; j0 < cnt; j0 += R * BT) {
+        int w[R], s[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int j = j0 + r * BT + tid;
+          w[r] = j < cnt ? stage[j] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) s[r] = w[r] >= 0 ? lab_ld(&ns[w[r]]) : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (w[r] >= 0) vstat = hook_edge(vstat, w[r], s[r], ns);
       }
     }
     __syncthreads();
@@ -401,23 +625,38 @@ template <bool BATCH>
 static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s, hipEvent_t* ev) {
   const long long vb = (p.nslots + 255) / 256;
   if (vb <= 0) return hipSuccess;
+  const long long words = (p.nslots + 63) / 64;
+  const long long sb = (words + kScanBlock - 1) / kScanBlock;
   const dim3 grid((unsigned)vb), block(256);
-  if (ev) (void)hipEventRecord(ev[0], s);
+  int k = 0;
+  auto mark_ev = [&]() {
+    if (ev) (void)hipEventRecord(ev[k], s);
+    k++;
+  };
+  mark_ev();
   k_init<BATCH><<<grid, block, 0, s>>>(p);
-  if (ev) (void)hipEventRecord(ev[1], s);
+  mark_ev();
   k_hook_thread<BATCH><<<grid, block, 0, s>>>(p);
-  if (ev) (void)hipEventRecord(ev[2], s);
+  mark_ev();
+  k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
+  mark_ev();
+  k_hub_top<<<dim3(1), dim3(1024), 0, s>>>(p);
+  mark_ev();
+  k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
+  mark_ev();
+  k_mark<BATCH><<<grid, block, 0, s>>>(p);
+  mark_ev();
   k_hook_wave<BATCH><<<grid, block, 0, s>>>(p);
-  if (ev) (void)hipEventRecord(ev[3], s);
+  mark_ev();
   const dim3 hgrid((unsigned)shape.hub_blocks);
   switch (shape.block_threads) {
     case 1024: k_hook_block<BATCH, 1024><<<hgrid, dim3(1024), 0, s>>>(p); break;
     case 512: k_hook_block<BATCH, 512><<<hgrid, dim3(512), 0, s>>>(p); break;
     default: k_hook_block<BATCH, 256><<<hgrid, dim3(256), 0, s>>>(p); break;
   }
-  if (ev) (void)hipEventRecord(ev[4], s);
+  mark_ev();
   k_flatten<BATCH><<<grid, block, 0, s>>>(p);
-  if (ev) (void)hipEventRecord(ev[5], s);
+  mark_ev();
   return hipGetLastError();
 }
 

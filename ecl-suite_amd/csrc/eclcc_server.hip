@@ -196,11 +196,11 @@ struct eclcc_server {
     int32_t* d_nidx = (int32_t*)carve(4 * (size_t)ib);
     int32_t* d_nlist = (int32_t*)carve(4 * (size_t)eb);
     int32_t* d_nstat = (int32_t*)carve(4 * (size_t)vb);
-    const size_t wbytes = workspace_bytes(vb, hub_cap);
+    const size_t wbytes = workspace_bytes(vb, hub_cap, G);
     void* d_work = carve(wbytes);
     if (off > b->dev_bytes) return set_error(ECLCC_ERR_STATE, "server batch layout overflow");
     Workspace w;
-    if (!carve_workspace(d_work, wbytes, vb, hub_cap, &w))
+    if (!carve_workspace(d_work, wbytes, vb, hub_cap, G, &w))
       return set_error(ECLCC_ERR_STATE, "server workspace carve failed");
     const hipMemcpyKind h2d = hipMemcpyHostToDevice;
     int rc = check_hip(hipMemcpyAsync(d_desc, desc.data(), sizeof(GraphDesc) * G, h2d, s),
@@ -221,17 +221,11 @@ struct eclcc_server {
     p.nidx = d_nidx;
     p.nlist = d_nlist;
     p.nstat = d_nstat;
+    p.ngraphs = G;
     p.nslots = vb;
     p.desc = d_desc;
     p.chunk_graph = d_chunk;
-    p.midbits = w.midbits;
-    p.hubs = w.hubs;
-    p.hubctr = w.hubctr;
-    p.flags = w.flags;
-    p.hub_cap = w.hub_cap;
-    p.thread_max = tuning.thread_max;
-    p.wave_max = tuning.wave_max;
-    p.hub_chunk = tuning.hub_chunk;
+    fill_params(&p, w, tuning);
     rc = check_hip(launch_pipeline(p, true, Shape{tuning.hub_blocks, tuning.block_threads}, s, nullptr),
                    ECLCC_ERR_LAUNCH, "kernel launch failed");
     for (int g = 0; g < G && !rc; g++) {
@@ -261,7 +255,7 @@ struct eclcc_server {
   }
 
   void run() {
-    hipSetDevice(device);
+    (void)hipSetDevice(device);
     for (;;) {
       // 1. retire finished batches
       bool busy = false;
@@ -298,7 +292,7 @@ struct eclcc_server {
             const long long vb2 = vb + ((long long)r->n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
             const long long cap2 = hub_cap + hub_capacity(r->m, tuning.wave_max, 1);
             const size_t req2 = req_bytes + request_device_bytes(r->n, r->m);
-            const size_t need_d = 4096 + req2 + workspace_bytes(vb2, cap2);
+            const size_t need_d = 4096 + req2 + workspace_bytes(vb2, cap2, (long long)nb->reqs.size() + 1);
             const size_t need_s = sbytes + align256(4 * (size_t)r->n);
             if (dev.used() + need_d > dev.capacity() || stage.used() + need_s > stage.capacity())
               break;
@@ -409,7 +403,7 @@ int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const
     return set_error(ECLCC_ERR_ARG, "bad request arguments");
   const size_t need = 4096 + request_device_bytes(nodes, edges) +
                       workspace_bytes(((long long)nodes + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
-                                      hub_capacity(edges, s->tuning.wave_max, 1));
+                                      hub_capacity(edges, s->tuning.wave_max, 1), 1);
   if (need > s->dev.capacity() || align256(4 * (size_t)nodes) > s->stage.capacity())
     return set_error(ECLCC_ERR_BUDGET, "request larger than the HBM budget");
   Request* r = new (std::nothrow) Request;

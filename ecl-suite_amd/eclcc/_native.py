@@ -34,7 +34,8 @@ class EclccParams(C.Structure):
         ("hub_chunk", i32),
         ("hub_blocks", i32),
         ("block_threads", i32),
-        ("reserved", i32 * 3),
+        ("giant_filter", i32),
+        ("reserved", i32 * 2),
     ]
 
     def as_dict(self) -> dict:
@@ -108,6 +109,8 @@ def cc_lib() -> C.CDLL:
         L.eclcc_status_string.argtypes = [C.c_int]
         L.eclcc_status_string.restype = C.c_char_p
         L.eclcc_version.restype = C.c_char_p
+        L.eclcc_kernel_name.argtypes = [C.c_int]
+        L.eclcc_kernel_name.restype = C.c_char_p
         L.eclcc_params_default.argtypes = [PP]
         L.eclcc_params_default.restype = None
         L.eclcc_compute.argtypes = [i32, i32, vp, vp, vp]

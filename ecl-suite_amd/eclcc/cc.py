@@ -6,7 +6,7 @@ ECL-CC-atomic.cu:275): host CSR in, labels written into ``nstat``.  Errors
 raise ``EclccError`` instead of ``exit(-1)``.
 
 ``DeviceGraph`` keeps the CSR, labels and workspace resident in HBM (torch
-tensors are only the allocator) and launches the five kernels on a stream.
+tensors are only the allocator) and launches the kernel sequence on a stream.
 Every call goes through libeclcc.so; nothing here computes labels itself.
 """
 from __future__ import annotations
@@ -19,7 +19,8 @@ import numpy as np
 from . import _native as N
 from .graph import Graph
 
-KERNELS = ("init", "compute1", "compute2", "compute3", "flatten")
+KERNELS = ("init", "compute1", "hubscan", "hubtop", "pick", "mark", "compute2", "compute3",
+           "flatten")
 
 
 def default_params(**overrides) -> N.EclccParams:
@@ -43,7 +44,7 @@ def version() -> str:
 def compute_cc(nodes: int, edges: int, nidx: np.ndarray, nlist: np.ndarray, nstat: np.ndarray,
                params: Optional[N.EclccParams] = None) -> float:
     """computeCC (ECL-CC-atomic.cu:275-325): fills nstat; returns the
-    device time of the five kernels in seconds."""
+    device time of the kernel sequence in seconds."""
     for a in (nidx, nlist, nstat):
         if a.dtype != np.int32 or not a.flags.c_contiguous:
             raise TypeError("CSR and label arrays must be contiguous int32")
@@ -103,9 +104,9 @@ class DeviceGraph:
                 C.byref(self.params), self._stream(stream)))
 
     def run_profiled(self, stream=None) -> Tuple[list, float]:
-        """Per-kernel milliseconds (init, compute1, compute2, compute3,
-        flatten) and the whole window, from HIP events on the launch stream."""
-        k = (C.c_float * 5)()
+        """Per-kernel milliseconds (in KERNELS order) and the whole window,
+        from HIP events on the launch stream."""
+        k = (C.c_float * len(KERNELS))()
         tot = C.c_float(0.0)
         with self.torch.cuda.device(self.device):
             N.check_cc(N.cc_lib().eclcc_run_device_profiled(

@@ -60,7 +60,10 @@ typedef struct eclcc_params {
   int32_t hub_chunk;      /* edges per block task for degree > wave_max_deg */
   int32_t hub_blocks;     /* blocks of the block-level kernel (0 = 8 per CU) */
   int32_t block_threads;  /* threads per block of the block-level kernel (256..1024) */
-  int32_t reserved[3];
+  int32_t giant_filter;   /* 1: after compute1, mark the vertices joined to each graph's
+                             largest sampled tree and skip edges with both ends marked
+                             (exact; saves label gathers on power-law graphs) */
+  int32_t reserved[2];
 } eclcc_params;
 
 /* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */
@@ -69,13 +72,13 @@ void eclcc_params_default(eclcc_params* p);
 /* ---- plain drop-in ----------------------------------------------------- */
 
 /* Replaces computeCC (ECL-CC-atomic.cu:275-325): host CSR in, host labels
-   out.  Allocates, copies, runs init/compute1/compute2/compute3/flatten,
+   out.  Allocates, copies, runs the kernel sequence (init ... flatten),
    copies back and frees, like the reference.  edges is the CSR length. */
 int eclcc_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
                   int32_t* nstat);
 
-/* Same, and *seconds = device time of the five kernels only (the window of
-   the reference's GPUTimer, ECL-CC-atomic.cu:305-312). params may be NULL. */
+/* Same, and *seconds = device time of the kernel sequence only (the window
+   of the reference's GPUTimer, ECL-CC-atomic.cu:305-312). params may be NULL. */
 int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
                         int32_t* nstat, const eclcc_params* params, double* seconds);
 
@@ -85,17 +88,21 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
    counters), in bytes. */
 size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params);
 
-/* Enqueues the five kernels on `stream` over device-resident CSR:
+/* Enqueues the kernel sequence on `stream` over device-resident CSR:
    d_nstat[nodes] receives the labels; d_work is workspace of at least
    eclcc_workspace_bytes() bytes.  Returns after launching (no sync). */
 int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                      const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
                      const eclcc_params* params, void* stream);
 
-/* As eclcc_run_device, but records HIP events around each of the five
-   kernels, waits, and writes their durations (ms) to kernel_ms[5]
-   (init, compute1, compute2, compute3, flatten) and the whole window to
-   *total_ms. */
+/* Kernels of one run, in launch order: init, compute1, hubscan, hubtop,
+   pick, mark, compute2, compute3, flatten. */
+#define ECLCC_NUM_KERNELS 9
+const char* eclcc_kernel_name(int k);
+
+/* As eclcc_run_device, but records HIP events around each kernel, waits,
+   and writes their durations (ms) to kernel_ms[ECLCC_NUM_KERNELS] and the
+   whole window to *total_ms. */
 int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                               const int32_t* d_nlist, int32_t* d_nstat, void* d_work,
                               size_t work_bytes, const eclcc_params* params, void* stream,
@@ -121,7 +128,7 @@ int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, i
      nindex_off_g in one packed int32 index array (n_g+1 local offsets), its
      nlist at nlist_off_g (int64 offsets: the batch may exceed 2^31 entries
      in total; each graph's own CSR stays int32).
-   The five kernels run once over all graphs.  Each graph's labels are the
+   The kernel sequence runs once over all graphs.  Each graph's labels are the
    same as eclcc_compute() on that graph alone. */
 typedef struct eclcc_batch eclcc_batch;
 
@@ -146,8 +153,8 @@ size_t eclcc_batch_device_bytes(const eclcc_batch* b);
 
 void eclcc_batch_free(eclcc_batch* b);
 
-/* Runs graph g of a batch ALONE through the plain device path (five kernel
-   launches for that graph) on `stream`: the one-graph-per-launch baseline
+/* Runs graph g of a batch ALONE through the plain device path (its own
+   kernel sequence) on `stream`: the one-graph-per-launch baseline
    the batched run is compared with, on the same resident data. */
 int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream);
 

@@ -1,0 +1,82 @@
+"""The C ABI boundary without a GPU: both libraries load and export every
+function include/*.h declares; argument errors come back as statuses; the
+CLI keeps the reference's usage/error behaviour; the kernels' label accesses
+all go through the relaxed-atomic helpers."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import eclcc
+from eclcc import _native as N
+
+
+@pytest.mark.parametrize("header,lib", [("eclcc.h", "libeclcc.so"), ("eclgraph.h", "libeclgraph.so")])
+def test_library_exports_every_declared_function(header, lib):
+    names = N.header_functions(header)
+    assert len(names) > 8
+    dll = C.CDLL(str(N.LIB_DIR / lib))
+    missing = [f for f in names if not hasattr(dll, f)]
+    assert not missing, f"{lib} lacks {missing}"
+
+
+def test_params_defaults_and_workspace():
+    p = eclcc.default_params()
+    assert p.thread_max_deg >= 0 and p.wave_max_deg > p.thread_max_deg
+    assert 64 <= p.hub_chunk <= 8192 and p.block_threads in (256, 512, 1024)
+    L = N.cc_lib()
+    small = L.eclcc_workspace_bytes(1000, 10000, C.byref(p))
+    big = L.eclcc_workspace_bytes(100000, 10000000, C.byref(p))
+    assert 0 < small < big
+    assert L.eclcc_workspace_bytes(0, 10, C.byref(p)) == 0
+    bad = eclcc.default_params(block_threads=300)
+    assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
+
+
+def test_argument_errors_are_statuses():
+    L = N.cc_lib()
+    rc = L.eclcc_run_device(0, 0, None, None, None, None, 0, None, None)
+    assert rc == -1 and b"too low" in L.eclcc_last_error()
+    rc = L.eclcc_compute(5, -1, None, None, None)
+    assert rc == -1
+    with pytest.raises(eclcc.EclccError):
+        eclcc.compute_cc(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    with pytest.raises(ValueError):
+        eclcc.compute_cc(3, 0, np.zeros(3, np.int32), np.zeros(0, np.int32), np.zeros(3, np.int32))
+    assert L.eclcc_status_string(-6) == b"request exceeds the HBM budget"
+    assert b"gfx950" in L.eclcc_version()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(N, "LIB_DIR", tmp_path)
+    monkeypatch.setattr(N, "_cc", None)
+    with pytest.raises(N.NativeLibraryMissing):
+        N.cc_lib()
+
+
+def test_cli_usage_and_file_errors(tmp_path):
+    exe = N.BIN_DIR / "eclcc"
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 255 and "USAGE:" in r.stderr
+    r = subprocess.run([str(exe), str(tmp_path / "none.egr")], capture_output=True, text=True)
+    assert r.returncode == 255 and "ERROR: could not open file" in r.stderr
+
+
+def test_label_accesses_go_through_atomic_helpers():
+    src = (N.PKG_ROOT / "csrc" / "eclcc_kernels.hip").read_text()
+    body = re.sub(r"//[^\n]*", "", src)
+    uses = [m for m in re.finditer(r"\b(ns|nstat)\[", body)]
+    assert uses
+    plain = []
+    for m in uses:
+        before = body[max(0, m.start() - 10):m.start()]
+        if re.search(r"lab_(ld|st|cas)\(&(?:G\.)?$", before):
+            continue
+        line = body[body.rfind("\n", 0, m.start()) + 1:body.find("\n", m.end())]
+        plain.append(line.strip())
+    # the one plain store is init's, where no other thread reads labels
+    assert plain == ["if (live) G.nstat[v] = m;"], plain

@@ -62,6 +62,8 @@ PARAMS = [
     dict(thread_max_deg=64, wave_max_deg=64, block_threads=512, hub_chunk=8192),
     dict(thread_max_deg=16, wave_max_deg=704, block_threads=1024, hub_blocks=3),
     dict(thread_max_deg=1 << 20, wave_max_deg=1 << 20),  # all at lane granularity
+    dict(giant_filter=0),
+    dict(giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=128),
 ]
 
 
@@ -103,7 +105,7 @@ def test_device_runs_are_repeatable(cuda):
             check(g, lab)
         assert np.array_equal(lab, first)
     kms, tot = dg.run_profiled()
-    assert len(kms) == 5 and tot > 0
+    assert len(kms) == 9 and tot > 0
     assert np.array_equal(dg.labels(), first)
 
 
@@ -127,3 +129,12 @@ def test_full_size_rmat24(cuda):
     check(g, lab)
     hubs, tasks = dg.hub_stats()
     assert hubs > 0 and tasks >= hubs
+
+
+def test_filter_on_and_off_agree_full_rmat20(cuda):
+    g = workloads.rmat(20, 16, 31)
+    announce(f"filter on/off {g.name}")
+    on = compute(g, default_params(giant_filter=1))
+    off = compute(g, default_params(giant_filter=0))
+    assert np.array_equal(on, off)
+    check(g, on)

@@ -1,0 +1,159 @@
+"""libeclgraph.so: .egr format compatibility with the reference reader and
+writer, and the properties of the seeded generators."""
+from __future__ import annotations
+
+import subprocess
+
+import numpy as np
+import pytest
+
+from eclcc import EclgraphError, Graph, workloads
+from eclcc.graph import rand64, set_generator_threads
+from oracle import oracle
+
+
+def assert_clean_symmetric(g, sorted_lists=True):
+    n = g.nodes
+    assert g.nindex[0] == 0 and g.nindex[-1] == g.edges
+    assert np.all(np.diff(g.nindex) >= 0)
+    assert g.nlist.min(initial=0) >= 0 and g.nlist.max(initial=0) < n
+    src = np.repeat(np.arange(n), np.diff(g.nindex))
+    assert not np.any(src == g.nlist), "self-loop"
+    fwd = src.astype(np.int64) * n + g.nlist
+    bwd = g.nlist.astype(np.int64) * n + src
+    assert np.array_equal(np.sort(fwd), np.sort(bwd)), "not symmetric"
+    if sorted_lists:
+        same = src[1:] == src[:-1]
+        assert np.all(g.nlist[1:][same] > g.nlist[:-1][same]), "unsorted or duplicate"
+
+
+def test_er_exact_pairs_and_clean():
+    g = Graph.er(5000, 20000, 3)
+    assert g.nodes == 5000 and g.edges == 40000
+    assert_clean_symmetric(g)
+
+
+def test_er65536_is_config_size():
+    g = workloads.er65536(1)
+    assert (g.nodes, g.edges) == (65536, 524288)
+    assert_clean_symmetric(g)
+    assert g.algorithmic_bytes() == 4 * 65537 + 4 * 524288 + 4 * 65536
+
+
+def test_grid_edge_counts():
+    s = 11
+    full = Graph.grid3d(s, s, s, 1.0, 1, permute=True)
+    assert full.edges == 2 * 3 * s * s * (s - 1)
+    assert_clean_symmetric(full)
+    assert Graph.grid3d(s, s, s, 0.0, 1).edges == 0
+    part = workloads.grid3d(20, 0.6, 2)
+    expect = 2 * 3 * 20 * 20 * 19 * 0.6
+    assert abs(part.edges - expect) < 0.05 * expect
+    assert_clean_symmetric(part)
+    # unpermuted grid: neighbours of (0,0,0) are 1, s, s*s
+    plain = Graph.grid3d(s, s, s, 1.0, 1, permute=False)
+    assert plain.nlist[plain.nindex[0]:plain.nindex[1]].tolist() == [1, s, s * s]
+
+
+def test_rmat_clean_and_deterministic():
+    a = workloads.rmat(13, 16, 7)
+    assert_clean_symmetric(a)
+    set_generator_threads(1)
+    try:
+        b = workloads.rmat(13, 16, 7)
+    finally:
+        set_generator_threads(0)
+    assert np.array_equal(a.nindex, b.nindex) and np.array_equal(a.nlist, b.nlist)
+    c = workloads.rmat(13, 16, 8)
+    assert not (a.edges == c.edges and np.array_equal(a.nlist, c.nlist))
+    d = a.degrees()
+    assert d.max() > 20 * d.mean(), "R-MAT degrees should be skewed"
+
+
+def test_rmat_non_power_of_two():
+    g = Graph.rmat(12, 8 * 3000, 5, n=3000)
+    assert g.nodes == 3000
+    assert_clean_symmetric(g)
+
+
+def test_batch_mix_spec():
+    from eclcc.workloads import mix_spec
+
+    specs = [mix_spec(i) for i in range(2000)]
+    ns = np.array([s.n for s in specs])
+    assert ns.min() >= 1000 and ns.max() <= 1_000_000
+    # log-uniform: median near sqrt(1e3*1e6) ~ 31623
+    assert 20000 < np.median(ns) < 50000
+    kinds = [s.kind for s in specs]
+    assert 0.45 < kinds.count("er") / len(kinds) < 0.55
+    assert mix_spec(17) == mix_spec(17)
+
+
+def test_rand64_is_splitmix64():
+    # splitmix64 from state 0: first output is 0xE220A8397B1DCDAF
+    assert rand64(0, 0) == 0xE220A8397B1DCDAF
+
+
+def test_egr_roundtrip(tmp_path):
+    g = workloads.rmat(10, 8, 2)
+    p = tmp_path / "g.egr"
+    g.write(p)
+    h = Graph.read(p)
+    assert np.array_equal(g.nindex, h.nindex) and np.array_equal(g.nlist, h.nlist)
+    assert h.eweight is None
+    raw = np.fromfile(p, dtype="<i4")
+    assert raw[0] == g.nodes and raw[1] == g.edges and raw.size == 2 + g.nodes + 1 + g.edges
+    g.eweight = np.arange(g.edges, dtype=np.int32)
+    g.write(p)
+    assert np.array_equal(Graph.read(p).eweight, g.eweight)
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(EclgraphError):
+        Graph.read(tmp_path / "missing.egr")
+    p = tmp_path / "bad.egr"
+    np.array([0, 0, 0], dtype="<i4").tofile(p)
+    with pytest.raises(EclgraphError):
+        Graph.read(p)  # nodes < 1
+    np.array([4, 10, 0, 1, 2], dtype="<i4").tofile(p)
+    with pytest.raises(EclgraphError):
+        Graph.read(p)  # truncated index
+
+
+@pytest.mark.skipif(not oracle.REF_ROUNDTRIP.exists(), reason="reference ECLgraph.h driver not built")
+def test_reference_reader_writer_interop(tmp_path):
+    """Files written here are read by the reference's readECLgraph, and what
+    the reference's writeECLgraph writes back is byte-identical."""
+    for g in (workloads.grid3d(8, 0.5, 1), workloads.rmat(11, 16, 3)):
+        ours = tmp_path / "ours.egr"
+        theirs = tmp_path / "theirs.egr"
+        g.write(ours)
+        out = subprocess.run([str(oracle.REF_ROUNDTRIP), str(ours), str(theirs)], check=True,
+                             capture_output=True, text=True).stdout.split()
+        assert int(out[0]) == g.nodes and int(out[1]) == g.edges and out[2] == "0"
+        assert ours.read_bytes() == theirs.read_bytes()
+        back = Graph.read(theirs)
+        assert np.array_equal(back.nlist, g.nlist)
+
+
+def test_mtx_reader(tmp_path):
+    p = tmp_path / "m.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n"
+                 "5 5 4\n2 1\n3 2\n5 4\n4 4\n")
+    g = Graph.read_mtx(p)
+    assert g.nodes == 5
+    assert_clean_symmetric(g)
+    assert oracle.cc_labels(g.nindex, g.nlist).tolist() == [0, 0, 0, 3, 3]
+    q = tmp_path / "r.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n3 4 2\n1 4 2.5\n3 2 1e3\n")
+    h = Graph.read_mtx(q)
+    assert h.nodes == 4
+    assert_clean_symmetric(h)
+
+
+def test_from_edges_flags():
+    g = Graph.from_edges(4, [0, 1, 1, 2, 3], [1, 0, 1, 2, 2], flags=0)
+    assert g.edges == 5  # kept as given
+    h = Graph.from_edges(4, [0, 1, 1, 2, 3], [1, 0, 1, 2, 2])
+    assert_clean_symmetric(h)
+    assert h.edges == 4
