@@ -64,12 +64,14 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
     return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
   if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
   if (t.hub_chunk < 64 || t.hub_chunk > kMaxHubChunk)
-    return set_error(ECLCC_ERR_ARG, "hub_chunk must be in [64, 8192]");
+    return set_error(ECLCC_ERR_ARG, "hub_chunk must be in [64, 65536]");
   if (t.block_threads != 256 && t.block_threads != 512 && t.block_threads != 1024)
     return set_error(ECLCC_ERR_ARG, "block_threads must be 256, 512 or 1024");
   if (t.hub_blocks <= 0) {
+    // one residency wave: each block owns a contiguous task range, so a
+    // second wave of blocks would start late and finish last
     const int cus = device_cu_count();
-    t.hub_blocks = 8 * (cus > 0 ? cus : 256);
+    t.hub_blocks = (cus > 0 ? cus : 256) * (cus > 0 ? hub_block_residency(t.block_threads) : 4);
   }
   *out = t;
   return ECLCC_OK;
@@ -225,7 +227,7 @@ void eclcc_params_default(eclcc_params* p) {
   if (!p) return;
   std::memset(p, 0, sizeof *p);
   p->thread_max_deg = 16;
-  p->wave_max_deg = 704;  // 11 x 64-lane wavefronts
+  p->wave_max_deg = 2048;  // 32 x 64-lane wavefronts (measured best on R-MAT-24)
   p->hub_chunk = 4096;
   p->hub_blocks = 0;
   p->block_threads = 256;

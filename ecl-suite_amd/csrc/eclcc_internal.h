@@ -14,8 +14,10 @@ namespace eclcc {
 // Vertex slots of a batch are padded per graph to this multiple, so every
 // 256-slot chunk (and every 64-slot wavefront) belongs to exactly one graph.
 constexpr int kSlotAlign = 256;
-// Largest hub_chunk the LDS staging buffer of the block-level kernel holds.
-constexpr int kMaxHubChunk = 8192;
+// Largest hub_chunk (edges per block task of the block-level kernel).
+constexpr int kMaxHubChunk = 65536;
+// Adjacency entries each thread of the block-level kernel loads per round.
+constexpr int kHubPerThread = 8;
 // 64-slot words per block of the hub-list scan.
 constexpr int kScanBlock = 1024;
 // Vertices sampled per graph to find its largest component.
@@ -31,10 +33,13 @@ struct GraphDesc {
   int m;            // CSR entries
 };
 
-// High-degree task list entry: vertex slot and the index of its first edge task.
+// High-degree task list entry: vertex slot, index of its first edge task and
+// the bounds of its adjacency list (local nindex values).
 struct HubEntry {
   long long slot;
   long long first_task;
+  int beg;
+  int end;
 };
 
 // Everything the kernels read; passed by value.
@@ -99,6 +104,8 @@ void fill_params(KParams* p, const Workspace& w, const Tuning& t);
 hipError_t launch_pipeline(const KParams& p, bool batch, const Shape& shape, hipStream_t s,
                            hipEvent_t* ev);
 hipError_t launch_init_only(const KParams& p, hipStream_t s);
+// Blocks of the block-level kernel one CU holds at once (occupancy API).
+int hub_block_residency(int block_threads);
 
 // Status helpers (thread-local message).
 int set_error(int code, const std::string& msg);

@@ -118,6 +118,11 @@ __device__ __forceinline__ long long wave_slot() {
 
 // ---- find and hook ----------------------------------------------------------
 
+// Vertex 0 is a root for good: no ID is smaller, so no CAS can swing it and
+// no pointer jump writes to a root.  Walks stop there without reading its
+// label, which every thread of a graph whose largest component holds vertex
+// 0 would otherwise read.
+
 // Root of idx given curr = label[idx] already loaded; on the way every
 // visited vertex is re-pointed to its grandparent ("intermediate pointer
 // jumping", ECL-CC-atomic.cu:92-104).  Writes only move a pointer to an
@@ -125,7 +130,7 @@ __device__ __forceinline__ long long wave_slot() {
 __device__ __forceinline__ int find_from(int idx, int curr, int* nstat) {
   if (curr != idx) {
     int prev = idx, next;
-    while (curr > (next = lab_ld(&nstat[curr]))) {
+    while (curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
       lab_st(&nstat[prev], next);
       prev = curr;
       curr = next;
@@ -145,7 +150,7 @@ __device__ __forceinline__ int find_root(int idx, int* nstat) {
 __device__ __forceinline__ int find_until(int idx, int curr, int stop, int* nstat) {
   if (curr != idx) {
     int prev = idx, next;
-    while (curr != stop && curr > (next = lab_ld(&nstat[curr]))) {
+    while (curr != stop && curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
       lab_st(&nstat[prev], next);
       prev = curr;
       curr = next;
@@ -425,12 +430,17 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
     const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
     const bool hub = (hb >> lane) & 1ull;
     unsigned tasks = 0;
-    if (hub) tasks = (unsigned)((G.nidx[v + 1] - G.nidx[v] + p.hub_chunk - 1) / p.hub_chunk);
+    int hbeg = 0, hend = 0;
+    if (hub) {
+      hbeg = G.nidx[v];
+      hend = G.nidx[v + 1];
+      tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
+    }
     const unsigned excl = wave_incl_sum(tasks) - tasks;
     if (hub) {
       const long long idx = (long long)(unsigned)base + lanes_below(hb);
       if (idx < p.hub_cap) {
-        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl};
+        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl, hbeg, hend};
       } else {
         __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -452,110 +462,183 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
 // looked at, and processes all medium-degree vertices among them together:
 // their adjacency lists are laid end to end and dealt out evenly over the
 // 64 lanes, four entries per lane per step, so a wavefront never idles on a
-// short list.  An entry's vertex is found by a binary search over the lanes'
-// list offsets (cross-lane shuffles); its root estimate comes from the lane
-// that owns the vertex, which found it once for all of its entries.  Entries
-// whose two ends are both marked members of the largest tree are skipped
-// after one bitmap lookup.
+// short list.  The owner of each entry comes from a per-wavefront LDS table
+// (one byte per entry, filled list by list for a window of kWaveWindow
+// entries); list bases and root estimates of the 64 lanes sit beside it.
+// A vertex already marked as joined to the sampled root uses that root as
+// its estimate without walking.  Entries whose two ends are both marked are
+// skipped after one bitmap lookup.
+
+constexpr int kWaveWindow = 2048;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
+  __shared__ unsigned char s_owner[4][kWaveWindow];
+  __shared__ int s_base[4][64];
+  __shared__ int s_root[4][64];
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const unsigned long long bits = p.midbits[ws >> 6];
+  const unsigned long long wbits = p.midbits[ws >> 6];
+  const unsigned long long bits =
+      ((unsigned long long)uniform((int)(wbits >> 32)) << 32) | (unsigned)uniform((int)wbits);
   if (!bits) return;
   const GraphRef G = resolve<BATCH>(p, ws);
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
   const int lane = lane_id();
+  const int wv = uniform((int)(threadIdx.x >> 6));
+  unsigned char* const owner = s_owner[wv];
   const bool mine = (bits >> lane) & 1ull;
   const int v = (int)(ws + lane - G.slot0);
   const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
+  const int groot = gw ? p.giant[G.g] : -1;
+  // a marked vertex is joined to the sampled root: that root is a valid
+  // estimate (the CAS of a hook checks it), no walk needed
+  const bool marked = (gw >> lane) & 1ull;
   int beg = 0, deg = 0, root = 0;
   if (mine) {
-    root = lab_ld(&ns[v]);
+    root = marked ? groot : lab_ld(&ns[v]);
     beg = G.nidx[v];
     deg = G.nidx[v + 1] - beg;
   }
-  if (mine) root = find_from(v, root, ns);
+  if (mine && !marked) root = find_from(v, root, ns);
   const int incl = (int)wave_incl_sum((unsigned)deg);
   const int off = incl - deg;
   const int total = __shfl(incl, 63);
+  s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
+  s_root[wv][lane] = root;
   const int vbase = (int)(ws - G.slot0);
-  for (int e0 = 0; e0 < total; e0 += 256) {
-    int u[4], src[4], vr[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int e = e0 + k * 64 + lane;
-      int sl = 0;  // last lane whose list starts at or before entry e
-#pragma unroll
-      for (int step = 32; step > 0; step >>= 1)
-        if (__shfl(off, sl + step) <= e) sl += step;
-      src[k] = sl;
-      const int lb = __shfl(beg, sl);
-      const int lo = __shfl(off, sl);
-      vr[k] = __shfl(root, sl);
-      u[k] = e < total ? ld_stream(&nl[lb + (e - lo)]) : INT_MAX;
+  for (int w0 = 0; w0 < total; w0 += kWaveWindow) {
+    const int wend = w0 + kWaveWindow < total ? w0 + kWaveWindow : total;
+    for (unsigned long long m = bits; m; m &= m - 1) {
+      const int L = __ffsll((long long)m) - 1;
+      const int oL = __builtin_amdgcn_readlane(off, L);
+      const int dL = __builtin_amdgcn_readlane(deg, L);
+      const int a = oL > w0 ? oL : w0;
+      const int b = oL + dL < wend ? oL + dL : wend;
+      for (int x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
     }
-    bool need[4];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int e0 = w0; e0 < wend; e0 += 256) {
+      int u[4], src[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) need[k] = vbase + src[k] > u[k];
-    if (gw) {
-      unsigned in[4];
+      for (int k = 0; k < 4; k++) {
+        const int e = e0 + k * 64 + lane;
+        const bool valid = e < wend;
+        src[k] = valid ? owner[e - w0] : 0;
+        u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
+      }
+      bool need[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) need[k] = vbase + src[k] > u[k];
+      if (gw) {
+        unsigned in[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          in[k] = (need[k] && ((gw >> src[k]) & 1ull)) ? slot_bit(p.gbits, G.slot0 + u[k]) : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++) need[k] = need[k] && !in[k];
+      }
+      int sv[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) sv[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        in[k] = (need[k] && ((gw >> src[k]) & 1ull)) ? slot_bit(p.gbits, G.slot0 + u[k]) : 0u;
-#pragma unroll
-      for (int k = 0; k < 4; k++) need[k] = need[k] && !in[k];
+        if (need[k]) (void)hook_edge(s_root[wv][src[k]], u[k], sv[k], ns);
     }
-    int s[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) s[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (need[k]) (void)hook_edge(vr[k], u[k], s[k], ns);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------
-// Blocks stride over the edge tasks of all high-degree vertices.  A task is
-// hub_chunk consecutive entries of one adjacency list: every thread issues
-// its share of the coalesced loads at once, the entries that still need a
-// hook (< v, and not both ends in the largest tree) are compacted into LDS
-// with wave ballots, and the compacted entries are hooked with all threads
-// busy and up to 16 label loads in flight per thread.  Adjacency lists are
-// sorted, so the bitmap lookups of one task fall on few cache lines.
+// Each block takes a contiguous range of the edge tasks of all high-degree
+// vertices (one search for its first task, then a forward walk of the task
+// list).  A task is hub_chunk consecutive entries of one adjacency list,
+// read in rounds of kHubPerThread coalesced loads per thread, all in flight
+// together.  The entries that still need a hook (< v, and not both ends in
+// the largest tree) are staged in LDS as (neighbour, root estimate of v)
+// pairs -- one LDS atomic per wavefront per round -- and kept there across
+// tasks; the block hooks them all together when the stage is about to fill
+// and at the end, with every thread busy and kHubPerThread label loads in
+// flight.  So the streaming part of a task never waits for label chains.
+
+template <int R, int BT>
+__device__ __forceinline__ void flush_stage(const int2* stage, int cnt, int* ns) {
+  for (int j0 = 0; j0 < cnt; j0 += R * BT) {
+    int w[R], vs[R], sv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int j = j0 + r * BT + (int)threadIdx.x;
+      const int2 e = j < cnt ? stage[j] : make_int2(-1, 0);
+      w[r] = e.x;
+      vs[r] = e.y;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) sv[r] = w[r] >= 0 ? lab_ld(&ns[w[r]]) : 0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (w[r] >= 0) (void)hook_edge(vs[r], w[r], sv[r], ns);
+  }
+}
 
 template <bool BATCH, int BT>
 __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
-  constexpr int R = 16;
-  __shared__ int stage[kMaxHubChunk];
+  constexpr int R = kHubPerThread;
+  constexpr int CAP = 2 * R * BT;  // staged pairs; room for one more round when below R*BT
+  __shared__ int2 stage[CAP];
   __shared__ int s_cnt;
   const unsigned long long ctr = *p.hubctr;
   const long long pushed = (long long)(unsigned)ctr;
   const long long nhubs = pushed < p.hub_cap ? pushed : p.hub_cap;
   const long long ntasks = (long long)(ctr >> 32);
+  if (nhubs <= 0) return;
+  // each block takes one contiguous range of tasks
+  const long long per = (ntasks + gridDim.x - 1) / gridDim.x;
+  const long long t0 = (long long)blockIdx.x * per;
+  const long long t1 = t0 + per < ntasks ? t0 + per : ntasks;
+  if (t0 >= t1) return;
   const int tid = threadIdx.x;
   const int lane = lane_id();
-  for (long long t = blockIdx.x; t < ntasks; t += gridDim.x) {
-    // last hub whose first task is <= t
-    long long lo = 0, hi = nhubs - 1;
-    while (lo < hi) {
-      const long long mid = (lo + hi + 1) >> 1;
-      if (p.hubs[mid].first_task <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const HubEntry h = p.hubs[lo];
+  if (tid == 0) s_cnt = 0;
+  long long k = 0, hi = nhubs - 1;  // last hub whose first task is <= t0
+  while (k < hi) {
+    const long long mid = (k + hi + 1) >> 1;
+    if (p.hubs[mid].first_task <= t0) k = mid;
+    else hi = mid - 1;
+  }
+  int* ns = nullptr;
+  for (long long t = t0; t < t1; t++) {
+    while (k + 1 < nhubs && p.hubs[k + 1].first_task <= t) k++;
+    const HubEntry h = p.hubs[k];
     const GraphRef G = resolve<BATCH>(p, h.slot);
+    if (G.nstat != ns) {  // staged pairs belong to the previous graph: hook them first
+      __syncthreads();
+      const int cnt = s_cnt;
+      if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns);
+      __syncthreads();
+      if (tid == 0) s_cnt = 0;
+      ns = G.nstat;
+    }
     const int v = (int)(h.slot - G.slot0);
     const bool vin = p.filter && slot_bit(p.gbits, h.slot);
-    const long long cb = (long long)G.nidx[v] + (t - h.first_task) * (long long)p.hub_chunk;
-    const long long ce_full = G.nidx[v + 1];
+    const long long cb = (long long)h.beg + (t - h.first_task) * (long long)p.hub_chunk;
     const int cbeg = (int)cb;
-    const int cend = (int)(cb + p.hub_chunk < ce_full ? cb + p.hub_chunk : ce_full);
-    if (tid == 0) s_cnt = 0;
-    __syncthreads();
+    const int cend = (int)(cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end);
+    // root estimate of v: the sampled root if v is marked, else found once
+    const int vstat = vin ? p.giant[G.g] : find_root(v, G.nstat);
     for (int base = cbeg; base < cend; base += R * BT) {
+      __syncthreads();
+      if (s_cnt > CAP - R * BT) {
+        const int cnt = s_cnt;
+        flush_stage<R, BT>(stage, cnt, ns);
+        __syncthreads();
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+      }
       int u[R];
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -565,39 +648,26 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
       unsigned in[R];
 #pragma unroll
       for (int r = 0; r < R; r++) in[r] = (vin && u[r] < v) ? slot_bit(p.gbits, G.slot0 + u[r]) : 0u;
+      unsigned keep = 0;
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const bool keep = u[r] < v && !in[r];
-        const unsigned long long mask = __ballot(keep);
-        if (mask) {
-          int off = 0;
-          if (lane == 0) off = atomicAdd(&s_cnt, __popcll(mask));
-          off = __shfl(off, 0);
-          if (keep) stage[off + lanes_below(mask)] = u[r];
-        }
-      }
-    }
-    __syncthreads();
-    const int cnt = s_cnt;
-    if (cnt > 0) {
-      int* const ns = G.nstat;
-      int vstat = find_root(v, ns);
-      for (int j0 = 0; j0 < cnt; j0 += R * BT) {
-        int w[R], s[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const int j = j0 + r * BT + tid;
-          w[r] = j < cnt ? stage[j] : -1;
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) s[r] = w[r] >= 0 ? lab_ld(&ns[w[r]]) : 0;
+      for (int r = 0; r < R; r++) keep |= (u[r] < v && !in[r]) ? (1u << r) : 0u;
+      const unsigned c = __popc(keep);
+      const unsigned incl = wave_incl_sum(c);
+      const unsigned wtot = __shfl(incl, 63);
+      if (wtot) {
+        int base_w = 0;
+        if (lane == 0) base_w = atomicAdd(&s_cnt, (int)wtot);
+        base_w = __shfl(base_w, 0);
+        int pos = base_w + (int)(incl - c);
 #pragma unroll
         for (int r = 0; r < R; r++)
-          if (w[r] >= 0) vstat = hook_edge(vstat, w[r], s[r], ns);
+          if (keep & (1u << r)) stage[pos++] = make_int2(u[r], vstat);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
+  const int cnt = s_cnt;
+  if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns);
 }
 
 // ---- flatten (ECL-CC-atomic.cu:250-264) ---------------------------------------
@@ -614,7 +684,7 @@ __global__ __launch_bounds__(256) void k_flatten(KParams p) {
     int s = lab_ld(&ns[v]);
     const int old = s;
     int next;
-    while (s > (next = lab_ld(&ns[s]))) s = next;
+    while (s != 0 && s > (next = lab_ld(&ns[s]))) s = next;
     if (s != old) lab_st(&ns[v], s);
   }
 }
@@ -663,6 +733,23 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
 hipError_t launch_pipeline(const KParams& p, bool batch, const Shape& shape, hipStream_t s,
                            hipEvent_t* ev) {
   return batch ? launch_all<true>(p, shape, s, ev) : launch_all<false>(p, shape, s, ev);
+}
+
+int hub_block_residency(int block_threads) {
+  int per_cu = 0;
+  hipError_t e;
+  switch (block_threads) {
+    case 1024:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hook_block<false, 1024>, 1024, 0);
+      break;
+    case 512:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hook_block<false, 512>, 512, 0);
+      break;
+    default:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hook_block<false, 256>, 256, 0);
+      break;
+  }
+  return e == hipSuccess && per_cu > 0 ? per_cu : 1;
 }
 
 hipError_t launch_init_only(const KParams& p, hipStream_t s) {

@@ -58,7 +58,7 @@ typedef struct eclcc_params {
   int32_t wave_max_deg;   /* degree <= this: one 64-lane wavefront per vertex
                              (reference: 352 = 11 x 32-lane warps, :122) */
   int32_t hub_chunk;      /* edges per block task for degree > wave_max_deg */
-  int32_t hub_blocks;     /* blocks of the block-level kernel (0 = 8 per CU) */
+  int32_t hub_blocks;     /* blocks of the block-level kernel (0 = as many as fit at once) */
   int32_t block_threads;  /* threads per block of the block-level kernel (256..1024) */
   int32_t giant_filter;   /* 1: after compute1, mark the vertices joined to each graph's
                              largest sampled tree and skip edges with both ends marked
