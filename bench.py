@@ -159,6 +159,17 @@ def main() -> None:
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    # batched / served
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--tune-graphs", type=int, default=256)
+    ap.add_argument("--tune-budget", type=float, default=60.0)
+    ap.add_argument("--requests", type=int, default=1024)
+    ap.add_argument("--qps", type=float, default=0.0, help="offered load; 0 = --load x capacity")
+    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -565,6 +565,49 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
                           b->d_nstat + d.vbase, b->d_work1, b->work1_bytes, &prm, stream);
 }
 
+int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) {
+  if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
+  Tuning t;
+  int rc = resolve_tuning(params, &t);
+  if (rc) return rc;
+  long long hub_cap = 0, max_n = 0, max_m = 0;
+  for (const GraphDesc& d : b->desc) {
+    hub_cap += hub_capacity(d.m, t.wave_max, 1);
+    max_n = std::max<long long>(max_n, d.n);
+    max_m = std::max<long long>(max_m, d.m);
+  }
+  const size_t wb = workspace_bytes(b->nslots, hub_cap, b->ngraphs);
+  if (wb != b->work_bytes) {
+    if ((rc = check_hip(hipDeviceSynchronize(), ECLCC_ERR_LAUNCH, "kernel execution failed"))) return rc;
+    (void)hipFree(b->d_work);
+    b->device_bytes -= b->work_bytes;
+    b->d_work = nullptr;
+    b->work_bytes = wb;
+    if ((rc = batch_alloc(b, &b->d_work, wb, "batch workspace"))) return rc;
+  }
+  if (!carve_workspace(b->d_work, b->work_bytes, b->nslots, hub_cap, b->ngraphs, &b->w))
+    return set_error(ECLCC_ERR_ALLOC, "workspace carve failed");
+  const size_t w1 = workspace_bytes(max_n, hub_capacity(max_m, t.wave_max, 1), 1);
+  if (w1 > b->work1_bytes) {
+    if ((rc = check_hip(hipDeviceSynchronize(), ECLCC_ERR_LAUNCH, "kernel execution failed"))) return rc;
+    (void)hipFree(b->d_work1);
+    b->device_bytes -= b->work1_bytes;
+    b->d_work1 = nullptr;
+    b->work1_bytes = w1;
+    if ((rc = batch_alloc(b, &b->d_work1, w1, "single-graph workspace"))) return rc;
+  }
+  b->tuning = t;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return ECLCC_OK;
+}
+
 int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat) {
   if (!b || !nstat || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
   const GraphDesc& d = b->desc[g];

@@ -122,6 +122,26 @@ struct PairArrays {
   void get(int64_t i, int32_t& u, int32_t& v) const { u = src[i]; v = dst[i]; }
 };
 
+// Degree counters: relaxed atomics when several threads build, plain
+// integers when one does (the batch generator builds many graphs at once,
+// one thread each).
+struct Counts {
+  bool atomic;
+  std::unique_ptr<std::atomic<int32_t>[]> a;
+  std::unique_ptr<int32_t[]> p;
+  explicit Counts(int32_t n, bool at) : atomic(at) {
+    if (at) a.reset(new (std::nothrow) std::atomic<int32_t>[(size_t)n]);
+    else p.reset(new (std::nothrow) int32_t[(size_t)n]);
+  }
+  bool ok() const { return atomic ? (bool)a : (bool)p; }
+  void set(int64_t x, int32_t v) {
+    if (atomic) a[x].store(v, std::memory_order_relaxed);
+    else p[x] = v;
+  }
+  int32_t get(int64_t x) const { return atomic ? a[x].load(std::memory_order_relaxed) : p[x]; }
+  int32_t inc(int64_t x) { return atomic ? a[x].fetch_add(1, std::memory_order_relaxed) : p[x]++; }
+};
+
 // Builds a CSR from npairs directed pairs. flags as in eclgraph_from_edges.
 // Counting sort by source, then per-list sort (+ unique) and compaction.
 template <class Src>
@@ -130,10 +150,10 @@ int build_csr(int32_t n, int64_t npairs, const Src& pairs, int flags, eclgraph_c
   const bool noself = flags & ECLGRAPH_DROP_SELF;
   const bool dedupe = flags & ECLGRAPH_DEDUPE;
 
-  auto cnt = std::unique_ptr<std::atomic<int32_t>[]>(new (std::nothrow) std::atomic<int32_t>[(size_t)n]);
-  if (!cnt) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (degree counts)");
+  Counts cnt(n, worker_count() > 1);
+  if (!cnt.ok()) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (degree counts)");
   parallel_for(n, 1 << 16, [&](int64_t lo, int64_t hi) {
-    for (int64_t x = lo; x < hi; x++) cnt[x].store(0, std::memory_order_relaxed);
+    for (int64_t x = lo; x < hi; x++) cnt.set(x, 0);
   });
   std::atomic<int> bad{0};
   parallel_for(npairs, 1 << 16, [&](int64_t lo, int64_t hi) {
@@ -142,8 +162,8 @@ int build_csr(int32_t n, int64_t npairs, const Src& pairs, int flags, eclgraph_c
       pairs.get(i, u, v);
       if ((uint32_t)u >= (uint32_t)n || (uint32_t)v >= (uint32_t)n) { bad.store(1); continue; }
       if (noself && u == v) continue;
-      cnt[u].fetch_add(1, std::memory_order_relaxed);
-      if (sym && u != v) cnt[v].fetch_add(1, std::memory_order_relaxed);
+      cnt.inc(u);
+      if (sym && u != v) cnt.inc(v);
     }
   });
   if (bad.load()) return fail(ECLGRAPH_ERR_ARG, "edge endpoint out of range [0, nodes)");
@@ -151,21 +171,21 @@ int build_csr(int32_t n, int64_t npairs, const Src& pairs, int flags, eclgraph_c
   auto off = alloc_array<int64_t>((int64_t)n + 1);
   if (!off) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (offsets)");
   off[0] = 0;
-  for (int32_t x = 0; x < n; x++) off[x + 1] = off[x] + cnt[x].load(std::memory_order_relaxed);
+  for (int32_t x = 0; x < n; x++) off[x + 1] = off[x] + cnt.get(x);
   const int64_t total = off[n];
 
   auto tmp = alloc_array<int32_t>(total);
   if (!tmp) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (edge buffer)");
   parallel_for(n, 1 << 16, [&](int64_t lo, int64_t hi) {
-    for (int64_t x = lo; x < hi; x++) cnt[x].store(0, std::memory_order_relaxed);
+    for (int64_t x = lo; x < hi; x++) cnt.set(x, 0);
   });
   parallel_for(npairs, 1 << 16, [&](int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; i++) {
       int32_t u, v;
       pairs.get(i, u, v);
       if (noself && u == v) continue;
-      tmp[off[u] + cnt[u].fetch_add(1, std::memory_order_relaxed)] = v;
-      if (sym && u != v) tmp[off[v] + cnt[v].fetch_add(1, std::memory_order_relaxed)] = u;
+      tmp[off[u] + cnt.inc(u)] = v;
+      if (sym && u != v) tmp[off[v] + cnt.inc(v)] = u;
     }
   });
 
@@ -176,7 +196,7 @@ int build_csr(int32_t n, int64_t npairs, const Src& pairs, int flags, eclgraph_c
       int32_t* e = tmp.get() + off[x + 1];
       std::sort(b, e);
       int64_t len = dedupe ? (std::unique(b, e) - b) : (e - b);
-      cnt[x].store((int32_t)len, std::memory_order_relaxed);
+      cnt.set(x, (int32_t)len);
     }
   });
 
@@ -189,7 +209,7 @@ int build_csr(int32_t n, int64_t npairs, const Src& pairs, int flags, eclgraph_c
   for (int32_t x = 0; x < n; x++) {
     if (acc > INT32_MAX) return fail(ECLGRAPH_ERR_RANGE, "edge count exceeds int32 CSR format");
     g->nindex[x] = (int32_t)acc;
-    acc += cnt[x].load(std::memory_order_relaxed);
+    acc += cnt.get(x);
   }
   if (acc > INT32_MAX) return fail(ECLGRAPH_ERR_RANGE, "edge count exceeds int32 CSR format");
   g->nindex[n] = (int32_t)acc;
@@ -412,26 +432,24 @@ int eclgraph_gen_rmat(int scale, int32_t n, int64_t samples, double a, double b,
   if ((int64_t)n > full) return fail(ECLGRAPH_ERR_ARG, "n exceeds 2^scale");
   if (samples < 0) return fail(ECLGRAPH_ERR_ARG, "samples must be >= 0");
   if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return fail(ECLGRAPH_ERR_ARG, "bad quadrant probabilities");
-  // quadrant thresholds on 32-bit draws
-  const double two32 = 4294967296.0;
-  const uint64_t ta = (uint64_t)std::llround(a * two32);
-  const uint64_t tab = (uint64_t)std::llround((a + b) * two32);
-  const uint64_t tabc = (uint64_t)std::llround((a + b + c) * two32);
-  const int draws = (scale + 1) / 2;  // 64-bit draws per sample (two levels each)
+  // quadrant thresholds on 21-bit draws; one 64-bit draw serves three levels
+  const double two21 = 2097152.0;
+  const uint32_t ta = (uint32_t)std::llround(a * two21);
+  const uint32_t tab = (uint32_t)std::llround((a + b) * two21);
+  const uint32_t tabc = (uint32_t)std::llround((a + b + c) * two21);
+  const int draws = (scale + 2) / 3;  // 64-bit draws per sample
   const uint64_t eseed = seed;
   const uint64_t pseed = mix64(seed ^ 0x5045524D55544531ULL);  // permutation stream
 
   std::vector<int32_t> src((size_t)samples), dst((size_t)samples);
   auto sample = [&](uint64_t trial, int32_t& su, int32_t& sv) {
     uint32_t x = 0, y = 0;
+    uint64_t r64 = 0;
     for (int l = 0; l < scale; l++) {
-      const uint64_t r64 = rand64(eseed, trial * (uint64_t)draws + (uint64_t)(l >> 1));
-      const uint64_t r = (l & 1) ? (r64 & 0xffffffffULL) : (r64 >> 32);
-      uint32_t bx, by;
-      if (r < ta) { bx = 0; by = 0; }
-      else if (r < tab) { bx = 0; by = 1; }
-      else if (r < tabc) { bx = 1; by = 0; }
-      else { bx = 1; by = 1; }
+      if (l % 3 == 0) r64 = rand64(eseed, trial * (uint64_t)draws + (uint64_t)(l / 3));
+      const uint32_t r = (uint32_t)(r64 >> (21 * (l % 3))) & 0x1FFFFFu;
+      const uint32_t bx = r >= tab ? 1u : 0u;
+      const uint32_t by = (r >= ta && r < tab) || r >= tabc ? 1u : 0u;
       x = (x << 1) | bx;
       y = (y << 1) | by;
     }

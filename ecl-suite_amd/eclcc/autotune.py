@@ -1,0 +1,102 @@
+"""Autotuner for the degree thresholds and launch shapes of the CC kernels.
+
+Coordinate descent over a small discrete space, timing the real batched run
+(hipGraph replay) on resident data with HIP events on the launch stream.
+Labels do not depend on any of these parameters (every setting computes the
+same minimum-ID labels); only the time changes.  Results are persisted as a
+JSON table keyed by workload class.
+"""
+from __future__ import annotations
+
+import json
+import statistics
+import time
+from pathlib import Path
+from typing import Dict, List, Optional, Tuple
+
+from . import _native as N
+from .batch import Batch
+from .cc import default_params
+
+SPACE: Dict[str, List[int]] = {
+    "wave_max_deg": [512, 2048, 8192],
+    "thread_max_deg": [8, 16, 32],
+    "hub_chunk": [2048, 4096, 8192],
+    "block_threads": [256, 512],
+    "giant_filter": [1, 0],
+}
+
+TABLE = N.PKG_ROOT / "tuning" / "mi355x.json"
+
+
+def _params(d: Dict[str, int]) -> N.EclccParams:
+    return default_params(**d)
+
+
+def time_batch(batch: Batch, params: N.EclccParams, reps: int = 3) -> float:
+    """Median milliseconds of one batched run (hipGraph replay)."""
+    import torch
+
+    batch.set_params(params)
+    stream = torch.cuda.current_stream()
+    batch.run(stream, use_graph=True)  # capture + warm
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        batch.run(stream, use_graph=True)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b))
+    return statistics.median(times)
+
+
+def tune(batch: Batch, space: Optional[Dict[str, List[int]]] = None, reps: int = 3,
+         passes: int = 2, budget_s: float = 120.0) -> Tuple[Dict[str, int], List[dict]]:
+    """Coordinate descent: each parameter in turn takes the value with the
+    lowest time while the others stay at their current best."""
+    space = space or SPACE
+    base = default_params().as_dict()
+    best = {k: base[k] for k in space}
+    cache: Dict[Tuple, float] = {}
+    trials: List[dict] = []
+    t_end = time.perf_counter() + budget_s
+
+    def measure(cfg: Dict[str, int]) -> float:
+        key = tuple(sorted(cfg.items()))
+        if key not in cache:
+            cache[key] = time_batch(batch, _params(cfg), reps)
+            trials.append({"params": dict(cfg), "ms": cache[key]})
+        return cache[key]
+
+    best_ms = measure(best)
+    for _ in range(passes):
+        changed = False
+        for name, values in space.items():
+            for v in values:
+                if time.perf_counter() > t_end:
+                    break
+                cfg = dict(best, **{name: v})
+                ms = measure(cfg)
+                if ms < best_ms * 0.995:
+                    best, best_ms, changed = cfg, ms, True
+        if not changed:
+            break
+    batch.set_params(_params(best))
+    return best, trials
+
+
+def save(key: str, params: Dict[str, int], trials: List[dict], path: Path = TABLE,
+         note: str = "") -> None:
+    path.parent.mkdir(parents=True, exist_ok=True)
+    table = json.loads(path.read_text()) if path.exists() else {}
+    table[key] = {"params": params, "trials": trials, "note": note}
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+
+
+def load(key: str, path: Path = TABLE) -> Optional[N.EclccParams]:
+    if not path.exists():
+        return None
+    entry = json.loads(path.read_text()).get(key)
+    return None if entry is None else _params(entry["params"])

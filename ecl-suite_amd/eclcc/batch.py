@@ -47,6 +47,10 @@ class Batch:
         s = None if stream is None else stream.cuda_stream
         N.check_cc(N.cc_lib().eclcc_batch_run_single(self.handle, int(g), s))
 
+    def set_params(self, params: N.EclccParams) -> None:
+        N.check_cc(N.cc_lib().eclcc_batch_set_params(self.handle, C.byref(params)))
+        self.params = params
+
     def labels(self, g: int) -> np.ndarray:
         out = np.empty(self.graphs[g].nodes, dtype=np.int32)
         N.check_cc(N.cc_lib().eclcc_batch_labels(self.handle, int(g), out.ctypes.data))

@@ -17,6 +17,27 @@ from . import _native as N
 SYMMETRIZE, DROP_SELF, DEDUPE = 1, 2, 4
 
 
+class _CsrOwner:
+    """Holds an eclgraph_csr handle; frees it when garbage collected."""
+
+    def __init__(self, h: C.c_void_p):
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.graph_lib().eclgraph_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _view(addr: int, count: int, owner: _CsrOwner) -> np.ndarray:
+    buf = (C.c_int32 * count).from_address(addr)
+    buf._owner = owner  # keeps the library memory alive as long as the array
+    return np.frombuffer(buf, dtype=np.int32)
+
+
 @dataclass
 class Graph:
     nindex: np.ndarray
@@ -44,21 +65,16 @@ class Graph:
 
     @staticmethod
     def _from_handle(h: C.c_void_p, name: str = "") -> "Graph":
+        """Wraps a library-owned CSR without copying; the library frees it
+        when the last array that views it is gone."""
         L = N.graph_lib()
-        try:
-            n = L.eclgraph_nodes(h)
-            m = L.eclgraph_edges(h)
-            ni = np.ctypeslib.as_array(C.cast(L.eclgraph_nindex(h), N.P_i32), shape=(n + 1,)).copy()
-            if m > 0:
-                nl = np.ctypeslib.as_array(C.cast(L.eclgraph_nlist(h), N.P_i32), shape=(m,)).copy()
-            else:
-                nl = np.zeros(0, dtype=np.int32)
-            wp = L.eclgraph_eweight(h)
-            w = None
-            if wp and m > 0:
-                w = np.ctypeslib.as_array(C.cast(wp, N.P_i32), shape=(m,)).copy()
-        finally:
-            L.eclgraph_free(h)
+        owner = _CsrOwner(h)
+        n = L.eclgraph_nodes(h)
+        m = L.eclgraph_edges(h)
+        ni = _view(L.eclgraph_nindex(h), n + 1, owner)
+        nl = _view(L.eclgraph_nlist(h), m, owner) if m > 0 else np.zeros(0, dtype=np.int32)
+        wp = L.eclgraph_eweight(h)
+        w = _view(wp, m, owner) if (wp and m > 0) else None
         return Graph(ni, nl, w, name)
 
     @staticmethod

@@ -94,10 +94,12 @@ def build_mix_graph(spec: MixSpec) -> Graph:
 
 
 def batch_mix(count: int, base_seed: int = BATCH_SEED, n_lo: int = 1000, n_hi: int = 1_000_000,
-              workers: Optional[int] = None) -> List[Graph]:
+              workers: Optional[int] = None, progress=None) -> List[Graph]:
     """`count` graphs of the batched mix, generated in parallel (each graph
-    single-threaded; ctypes releases the GIL)."""
+    single-threaded; ctypes releases the GIL).  progress(done, count) is
+    called from time to time."""
     import os
+    import time
 
     specs = [mix_spec(i, base_seed, n_lo, n_hi) for i in range(count)]
     workers = workers or min(16, os.cpu_count() or 1)
@@ -109,7 +111,11 @@ def batch_mix(count: int, base_seed: int = BATCH_SEED, n_lo: int = 1000, n_hi: i
     # largest first for load balance, results returned in index order
     order = sorted(range(count), key=lambda i: -specs[i].n)
     out: List[Optional[Graph]] = [None] * count
+    last = time.perf_counter()
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        for i, g in zip(order, ex.map(one, [specs[i] for i in order])):
+        for done, (i, g) in enumerate(zip(order, ex.map(one, [specs[i] for i in order])), 1):
             out[i] = g
+            if progress is not None and (time.perf_counter() - last > 10 or done == count):
+                progress(done, count)
+                last = time.perf_counter()
     return out  # type: ignore[return-value]

@@ -142,6 +142,11 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
    sequence is captured once into a hipGraph and replayed afterwards. */
 int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph);
 
+/* Replaces the batch's tuning parameters (used by the autotuner on resident
+   data).  Reallocates the workspace if its size changes and drops the
+   captured hipGraph; the next eclcc_batch_run recaptures it. */
+int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params);
+
 /* Copies graph g's labels (nodes[g] ints) to host memory (synchronous). */
 int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat);
 
