@@ -132,7 +132,7 @@ def plain(args) -> dict:
         "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
         "roofline": {
             "bound": "hbm",
-            "kernel": "ECL-CC pass (init, compute1, hubscan, hubtop, pick, mark, compute2, compute3, flatten)",
+            "kernel": "ECL-CC pass (init, sample, hubscan, hubtop, pick, mark, compute1, compute2, compute3, flatten)",
             "achieved": B / (pass_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -140,14 +140,41 @@ def plain(args) -> dict:
             "traffic": None,
         },
         "kernels_ms": kernel_ms,
+        "traffic_source": None,
         "pass_ms_events": pass_ms,
         "hub_vertices": hubs,
         "hub_tasks": tasks,
     }
+    tr = REPO / "profiles" / f"{args.workload}_traffic.json"
+    if tr.exists():
+        t = json.loads(tr.read_text())
+        if t.get("edges") == g.edges and t.get("nodes") == g.nodes:
+            out["roofline"]["traffic"] = t["bytes_per_pass"]
+            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, args.cpu_budget, labels)
     return out
+
+
+def copy_probe(args) -> dict:
+    """A plain device copy of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = 1 << 28  # 1 GiB of int32
+    a = torch.ones(n, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    return {"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms_per_copy": ms,
+            "GBps": 8 * n / (ms * 1e-3) / 1e9}
 
 
 def main() -> None:
@@ -155,7 +182,7 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served", "copy"])
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -178,6 +205,8 @@ def main() -> None:
         sys.exit(2)
     if args.mode == "plain":
         out = plain(args)
+    elif args.mode == "copy":
+        out = copy_probe(args)
     else:
         from eclcc import benchmodes
 

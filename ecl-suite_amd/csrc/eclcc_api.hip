@@ -60,6 +60,8 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   t.block_threads = p.block_threads;
   t.hub_blocks = p.hub_blocks;
   t.filter = p.giant_filter ? 1 : 0;
+  if (p.path_mode != 0 && p.path_mode != 1) return set_error(ECLCC_ERR_ARG, "path_mode must be 0 or 1");
+  t.path_mode = p.path_mode;
   if (t.thread_max < 0 || t.wave_max < t.thread_max)
     return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
   if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
@@ -86,9 +88,10 @@ size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs) {
   const size_t words = (size_t)((nslots + 63) / 64);
   const size_t blocks = (words + kScanBlock - 1) / kScanBlock;
   size_t b = 256;                                                   // counters, flags
-  b += 4 * align_up(sizeof(unsigned long long) * words, 256);       // mid, hub, info, member words
+  b += 5 * align_up(sizeof(unsigned long long) * words, 256);  // low, mid, hub, info, member words
   b += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
   b += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
+  b += align_up(sizeof(int) * (size_t)std::max<long long>(nslots, 1), 256);  // second entries
   b += align_up(sizeof(HubEntry) * (size_t)std::max<long long>(hub_cap, 1), 256);
   return b;
 }
@@ -103,7 +106,8 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
   w->hubctr = reinterpret_cast<unsigned long long*>(p);
   w->flags = reinterpret_cast<unsigned int*>(p + 128);
   p += 256;
-  unsigned long long** wordarrays[4] = {&w->midbits, &w->hubbits, &w->hubinfo, &w->gbits};
+  unsigned long long** wordarrays[5] = {&w->lowbits, &w->midbits, &w->hubbits, &w->hubinfo,
+                                        &w->gbits};
   for (auto a : wordarrays) {
     *a = reinterpret_cast<unsigned long long*>(p);
     p += align_up(sizeof(unsigned long long) * words, 256);
@@ -112,18 +116,22 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
   p += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
   w->giant = reinterpret_cast<int*>(p);
   p += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
+  w->second = reinterpret_cast<int*>(p);
+  p += align_up(sizeof(int) * (size_t)std::max<long long>(nslots, 1), 256);
   w->hubs = reinterpret_cast<HubEntry*>(p);
   w->hub_cap = std::max<long long>(hub_cap, 1);
   return true;
 }
 
 void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
+  p->lowbits = w.lowbits;
   p->midbits = w.midbits;
   p->hubbits = w.hubbits;
   p->hubinfo = w.hubinfo;
   p->gbits = w.gbits;
   p->blocktot = w.blocktot;
   p->giant = w.giant;
+  p->second = w.second;
   p->hubs = w.hubs;
   p->hubctr = w.hubctr;
   p->flags = w.flags;
@@ -132,6 +140,7 @@ void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
   p->wave_max = t.wave_max;
   p->hub_chunk = t.hub_chunk;
   p->filter = t.filter;
+  p->path_mode = t.path_mode;
 }
 
 }  // namespace eclcc
@@ -218,8 +227,9 @@ const char* eclcc_status_string(int s) {
 const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64)"; }
 
 const char* eclcc_kernel_name(int k) {
-  static const char* names[kNumKernels] = {"init", "compute1", "hubscan", "hubtop", "pick",
-                                           "mark", "compute2", "compute3", "flatten"};
+  static const char* names[kNumKernels] = {"init",     "sample",   "hubscan",  "hubtop",
+                                           "pick",     "mark",     "compute1", "compute2",
+                                           "compute3", "flatten"};
   return (k >= 0 && k < kNumKernels) ? names[k] : "";
 }
 
@@ -232,6 +242,7 @@ void eclcc_params_default(eclcc_params* p) {
   p->hub_blocks = 0;
   p->block_threads = 256;
   p->giant_filter = 1;
+  p->path_mode = 1;
 }
 
 size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) {
@@ -561,6 +572,7 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   prm.hub_blocks = b->tuning.hub_blocks;
   prm.block_threads = b->tuning.block_threads;
   prm.giant_filter = b->tuning.filter;
+  prm.path_mode = b->tuning.path_mode;
   return eclcc_run_device(d.n, d.m, b->d_nidx + d.ibase, b->d_nlist + d.ebase,
                           b->d_nstat + d.vbase, b->d_work1, b->work1_bytes, &prm, stream);
 }

@@ -53,12 +53,14 @@ struct KParams {
   const GraphDesc* desc;   // batch: graph table
   const int* chunk_graph;  // batch: graph of each 256-slot chunk
   // one 64-bit word per 64 slots
-  unsigned long long* midbits;   // medium-degree vertices (compute1 -> compute2)
-  unsigned long long* hubbits;   // high-degree vertices (compute1 -> mark)
+  unsigned long long* lowbits;   // low-degree vertices with a smaller neighbour (init -> compute1)
+  unsigned long long* midbits;   // medium-degree vertices (init -> compute2)
+  unsigned long long* hubbits;   // high-degree vertices (init -> mark)
   unsigned long long* hubinfo;   // hubs | tasks << 32 of the word, then its prefix in its block
   unsigned long long* gbits;     // members of the sampled largest component (mark -> compute2/3)
   unsigned long long* blocktot;  // per kScanBlock words: totals, then exclusive prefix
   int* giant;                    // per graph: sampled root of the largest component, or -1
+  int* second;                   // per slot: second list entry (init -> sample)
   HubEntry* hubs;                // high-degree vertices with their first task number
   unsigned long long* hubctr;    // low 32 bits: hubs; high 32: edge tasks
   unsigned int* flags;           // bit 0: hub list overflow
@@ -66,19 +68,22 @@ struct KParams {
   int thread_max;
   int wave_max;
   int hub_chunk;
-  int filter;  // 1: skip edges with both ends in the sampled largest component
+  int filter;     // 1: skip edges with both ends in the sampled largest component
+  int path_mode;  // 0: pointer jumping at every step (reference); 1: root jump
 };
 
 // Workspace carved out of one device buffer.
 struct Workspace {
   unsigned long long* hubctr;
   unsigned int* flags;
+  unsigned long long* lowbits;
   unsigned long long* midbits;
   unsigned long long* hubbits;
   unsigned long long* hubinfo;
   unsigned long long* gbits;
   unsigned long long* blocktot;
   int* giant;
+  int* second;
   HubEntry* hubs;
   long long hub_cap;
 };
@@ -90,7 +95,7 @@ struct Shape {
 
 // Resolved, validated parameters.
 struct Tuning {
-  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter;
+  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter, path_mode;
 };
 
 size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs);

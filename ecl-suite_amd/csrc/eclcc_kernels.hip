@@ -10,24 +10,25 @@
 // other thread reads labels.
 //
 // Kernel sequence (one launch each, no host synchronisation in between):
-//   init      labels := first smaller neighbour (reference init)
-//   compute1  degree <= thread_max: one lane per vertex; records medium and
-//             high-degree vertices as 64-bit ballot words (no atomics)
+//   init      labels := first smaller neighbour (reference init); degree
+//             binning into 64-bit ballot words (no atomics)
+//   sample    hooks of each vertex's second list entry
 //   hubscan   exclusive prefix of the high-degree counts per 1024-word block
 //   hubtop    prefix over the blocks; total hubs and edge tasks
 //   pick      per graph: the most frequent root among 256 sampled vertices
 //   mark      bitmap of the vertices already joined to that root; writes the
 //             high-degree task list in slot order
+//   compute1  low degree: lane granularity
 //   compute2  medium degree: wavefront granularity
 //   compute3  high degree: block granularity, edge tasks staged through LDS
 //   flatten   labels := root (reference flatten)
 //
 // Why the bitmap (MI355X measurements): random 4-byte gathers from the
 // 64 MB label array of a 2^24-vertex graph run at ~58 G/s (memory side),
-// from a 2 MB table that stays in L2 at ~265 G/s.  After compute1 most
-// medium- and high-degree edges of a power-law graph join two vertices that
-// are already in the largest tree; one bit per vertex (2 MB for 2^24) tells
-// that without touching the labels.  Skipping an edge whose ends are both
+// from a 2 MB table that stays in L2 at ~265 G/s.  After init and the
+// sampled hooks almost every remaining edge of random, grid and power-law
+// graphs joins two vertices that are already in the largest tree; one bit
+// per vertex (2 MB for 2^24) tells that without touching the labels.  Skipping an edge whose ends are both
 // connected to the sampled root is exact: they are in one component already.
 
 #include <climits>
@@ -123,37 +124,60 @@ __device__ __forceinline__ long long wave_slot() {
 // label, which every thread of a graph whose largest component holds vertex
 // 0 would otherwise read.
 
-// Root of idx given curr = label[idx] already loaded; on the way every
-// visited vertex is re-pointed to its grandparent ("intermediate pointer
-// jumping", ECL-CC-atomic.cu:92-104).  Writes only move a pointer to an
-// ancestor, i.e. to a smaller ID.
-__device__ __forceinline__ int find_from(int idx, int curr, int* nstat) {
+// Root of idx given curr = label[idx] already loaded.  Two ways of
+// shortening the path, chosen by KParams::path_mode:
+//   0  intermediate pointer jumping, as the reference's representative
+//      (ECL-CC-atomic.cu:92-104): every visited vertex is re-pointed to its
+//      grandparent, one label store per step;
+//   1  root jump: the walk only reads, and the start vertex is re-pointed to
+//      the root at the end, at most one store per walk.  On MI355X every
+//      agent-scope store is written through to the memory side; on random
+//      and grid graphs the stores of mode 0 cost more than the longer paths
+//      they save (measured: ER n=1M 1.33 -> 0.37 ms, 3-D grid 256^3 2.64 ->
+//      1.69 ms per pass).
+// Either way a store only moves a pointer to an ancestor, i.e. to a smaller
+// ID, and the labels that come out are the same.
+__device__ __forceinline__ int find_from(int idx, int curr, int* nstat, int mode) {
   if (curr != idx) {
-    int prev = idx, next;
-    while (curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
-      lab_st(&nstat[prev], next);
-      prev = curr;
-      curr = next;
+    if (mode == 0) {
+      int prev = idx, next;
+      while (curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
+        lab_st(&nstat[prev], next);
+        prev = curr;
+        curr = next;
+      }
+    } else {
+      const int first = curr;
+      int next;
+      while (curr != 0 && curr > (next = lab_ld(&nstat[curr]))) curr = next;
+      if (curr != first) lab_st(&nstat[idx], curr);
     }
   }
   return curr;
 }
 
-__device__ __forceinline__ int find_root(int idx, int* nstat) {
-  return find_from(idx, lab_ld(&nstat[idx]), nstat);
+__device__ __forceinline__ int find_root(int idx, int* nstat, int mode) {
+  return find_from(idx, lab_ld(&nstat[idx]), nstat, mode);
 }
 
 // Like find_from, but stops as soon as the walk reaches `stop` (a known
 // ancestor of the caller's vertex, usually the root of the largest tree):
 // the two are then joined, and the label of `stop` -- read by every thread
 // otherwise -- is not touched.
-__device__ __forceinline__ int find_until(int idx, int curr, int stop, int* nstat) {
+__device__ __forceinline__ int find_until(int idx, int curr, int stop, int* nstat, int mode) {
   if (curr != idx) {
-    int prev = idx, next;
-    while (curr != stop && curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
-      lab_st(&nstat[prev], next);
-      prev = curr;
-      curr = next;
+    if (mode == 0) {
+      int prev = idx, next;
+      while (curr != stop && curr != 0 && curr > (next = lab_ld(&nstat[curr]))) {
+        lab_st(&nstat[prev], next);
+        prev = curr;
+        curr = next;
+      }
+    } else {
+      const int first = curr;
+      int next;
+      while (curr != stop && curr != 0 && curr > (next = lab_ld(&nstat[curr]))) curr = next;
+      if (curr != first) lab_st(&nstat[idx], curr);
     }
   }
   return curr;
@@ -162,14 +186,28 @@ __device__ __forceinline__ int find_until(int idx, int curr, int stop, int* nsta
 // Min-linking of two root estimates (ECL-CC-atomic.cu:134-151): the larger
 // root is swung to the smaller by CAS; a failed CAS returns the value that
 // is there now, which is strictly smaller, and the walk continues from it.
-// Returns the caller's updated root estimate.
+// Each CAS is preceded by a plain relaxed load of the same label: on MI355X
+// device-scope atomics execute at the memory side and serialise per address,
+// and when many lanes hold the same just-merged root (dense random graphs,
+// grids) most of them learn from the load that it has moved and never issue
+// the atomic.  Returns the caller's updated root estimate.
 __device__ __forceinline__ int hook(int vstat, int ostat, int* nstat) {
   while (vstat != ostat) {
     if (vstat < ostat) {
+      const int cur = lab_ld(&nstat[ostat]);
+      if (cur != ostat) {  // already swung: follow it without an atomic
+        ostat = cur;
+        continue;
+      }
       const int seen = lab_cas(&nstat[ostat], ostat, vstat);
       if (seen == ostat) break;
       ostat = seen;
     } else {
+      const int cur = lab_ld(&nstat[vstat]);
+      if (cur != vstat) {
+        vstat = cur;
+        continue;
+      }
       const int seen = lab_cas(&nstat[vstat], vstat, ostat);
       if (seen == vstat) {
         vstat = ostat;  // v's tree now hangs below ostat
@@ -184,9 +222,9 @@ __device__ __forceinline__ int hook(int vstat, int ostat, int* nstat) {
 // Hooks the edge v-u (caller checked v > u); s = label[u] already loaded.
 // When u's path reaches v's root estimate the two are joined and nothing is
 // written.
-__device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat) {
+__device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, int mode) {
   if (s == vstat) return vstat;
-  const int r = find_until(u, s, vstat, nstat);
+  const int r = find_until(u, s, vstat, nstat, mode);
   if (r == vstat) return vstat;
   return hook(vstat, r, nstat);
 }
@@ -194,21 +232,28 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat) {
 // Hooks the neighbours u[0..K) of v that are smaller than v; all K label
 // loads are in flight before the first one is used.
 template <int K>
-__device__ __forceinline__ int hook_batch(int v, int vstat, const int (&u)[K], int* ns) {
+__device__ __forceinline__ int hook_batch(int v, int vstat, const int (&u)[K], int* ns, int mode) {
   int s[K];
 #pragma unroll
   for (int k = 0; k < K; k++) s[k] = v > u[k] ? lab_ld(&ns[u[k]]) : 0;
 #pragma unroll
   for (int k = 0; k < K; k++)
-    if (v > u[k]) vstat = hook_edge(vstat, u[k], s[k], ns);
+    if (v > u[k]) vstat = hook_edge(vstat, u[k], s[k], ns, mode);
   return vstat;
 }
 
-// ---- init (ECL-CC-atomic.cu:69-88) --------------------------------------------
+// ---- init (ECL-CC-atomic.cu:69-88) and degree binning ------------------------
 // label[v] = first neighbour smaller than v in list order, else v.  Each lane
 // checks the first four entries of its list; lists that need a longer scan
 // are finished cooperatively by the whole wavefront, 64 entries per step, the
 // first smaller entry being the lowest set lane of a ballot.
+// The same pass bins the vertices that have a smaller neighbour by degree
+// (the reference does this in compute1, :116-127): three 64-bit ballot words
+// per wavefront (low, medium, high; no shared worklist: one contended
+// device-scope atomic per push costs ~10 ns, serialised, on MI355X) plus the
+// count and edge-task total of the high-degree ones, and it keeps each
+// vertex's second list entry for the sampled hooks, so that nobody has to
+// fetch the head of every list a second time.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_init(KParams p) {
@@ -219,13 +264,13 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
-  int m = v, beg = 0, end = 0;
+  int m = v, beg = 0, end = 0, x1 = INT_MAX;
   bool longscan = false;
   if (live) {
     beg = G.nidx[v];
     end = G.nidx[v + 1];
     const int x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
-    const int x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
+    x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
     const int x2 = beg + 2 < end ? G.nlist[beg + 2] : INT_MAX;
     const int x3 = beg + 3 < end ? G.nlist[beg + 3] : INT_MAX;
     m = x0 < v ? x0 : x1 < v ? x1 : x2 < v ? x2 : x3 < v ? x3 : v;
@@ -250,39 +295,17 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
     }
     if (lane == src) m = found;
   }
-  if (live) G.nstat[v] = m;  // no other thread reads labels during init
-  if (slot == 0) {
-    *p.hubctr = 0ull;
-    *p.flags = 0u;
+  if (live) {
+    G.nstat[v] = m;  // no other thread reads labels during init
+    p.second[slot] = x1;
   }
-}
-
-// ---- compute1 (ECL-CC-atomic.cu:108-157) --------------------------------------
-// Degree binning and lane-granularity hooking.  Medium and high-degree
-// vertices are not pushed to a shared worklist (one contended device-scope
-// atomic per push costs ~10 ns, serialised, on MI355X): each wavefront stores
-// its two ballot words, and for the high-degree ones the count and the
-// number of edge tasks.
-
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
-  const long long ws = wave_slot();
-  if (ws >= p.nslots) return;
-  const GraphRef G = resolve<BATCH>(p, ws);
-  const int lane = lane_id();
-  const long long slot = ws + lane;
-  const int v = (int)(slot - G.slot0);
-  const bool live = slot < p.nslots && v < G.n;
-  int s0 = v, beg = 0, end = 0;
-  if (live) {  // independent loads, in flight together
-    s0 = lab_ld(&G.nstat[v]);
-    beg = G.nidx[v];
-    end = G.nidx[v + 1];
-  }
-  const bool active = live && s0 != v;  // roots after init have no smaller neighbour
+  // degree bins of the vertices that have a smaller neighbour
+  const bool active = live && m != v;
   const int deg = active ? end - beg : 0;
+  const bool low = active && deg <= p.thread_max;
   const bool mid = deg > p.thread_max && deg <= p.wave_max;
   const bool hub = deg > p.wave_max;
+  const unsigned long long lowmask = __ballot(low);
   const unsigned long long midmask = __ballot(mid);
   const unsigned long long hubmask = __ballot(hub);
   unsigned long long info = 0;
@@ -294,24 +317,93 @@ __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
   }
   if (lane == 0) {
     const long long w = ws >> 6;
+    p.lowbits[w] = lowmask;
     p.midbits[w] = midmask;
     p.hubbits[w] = hubmask;
     p.hubinfo[w] = info;
   }
+  if (slot == 0) {
+    *p.hubctr = 0ull;
+    *p.flags = 0u;
+  }
+}
 
-  if (active && deg <= p.thread_max) {
-    int* const ns = G.nstat;
-    const int* const nl = G.nlist;
-    // the first neighbours are fetched while v's root is being found
+// ---- sample: sampled hooks (first part of the reference compute1) -------------
+// One lane per vertex that has a smaller neighbour: hook its second list
+// entry if that is smaller too.  With init's pointer (the first smaller
+// entry) this links random, grid and power-law graphs into one large tree,
+// which mark then finds, so that the remaining edges can be checked against
+// the bitmap.  The sampled edge is hooked again later like every other edge;
+// re-hooking is a no-op.
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_sample(KParams p) {
+  const long long ws = wave_slot();
+  if (ws >= p.nslots) return;
+  const long long w = ws >> 6;
+  const unsigned long long act = p.lowbits[w] | p.midbits[w] | p.hubbits[w];
+  const int lane = lane_id();
+  if (!((act >> lane) & 1ull)) return;
+  const GraphRef G = resolve<BATCH>(p, ws);
+  const long long slot = ws + lane;
+  const int v = (int)(slot - G.slot0);
+  int* const ns = G.nstat;
+  const int u = p.second[slot];
+  const int s0 = lab_ld(&ns[v]);
+  if (u < v) {
+    const int su = lab_ld(&ns[u]);
+    const int vstat = find_from(v, s0, ns, p.path_mode);
+    (void)hook_edge(vstat, u, su, ns, p.path_mode);
+  }
+}
+
+// ---- compute1 (rest of ECL-CC-atomic.cu:108-157) -------------------------------
+// Low degree (<= thread_max): one lane per vertex hooks all entries smaller
+// than it, eight list entries in flight per step.  A vertex marked as joined
+// to the sampled root takes that root as its estimate and skips every entry
+// whose other end is marked too, after one bitmap lookup.
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
+  const long long ws = wave_slot();
+  if (ws >= p.nslots) return;
+  const unsigned long long bits = p.lowbits[ws >> 6];
+  if (!bits) return;
+  const GraphRef G = resolve<BATCH>(p, ws);
+  const int lane = lane_id();
+  if (!((bits >> lane) & 1ull)) return;
+  const long long slot = ws + lane;
+  const int v = (int)(slot - G.slot0);
+  int* const ns = G.nstat;
+  const int* const nl = G.nlist;
+  const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
+  const bool marked = (gw >> lane) & 1ull;
+  const int beg = G.nidx[v];
+  const int end = G.nidx[v + 1];
+  int vstat = marked ? p.giant[G.g] : -1;  // found on first use otherwise
+  for (int i = beg; i < end; i += 8) {
     int u[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) u[k] = beg + k < end ? ld_stream(&nl[beg + k]) : INT_MAX;
-    int vstat = find_from(v, s0, ns);
-    vstat = hook_batch<8>(v, vstat, u, ns);
-    for (int i = beg + 8; i < end; i += 8) {
+    for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : INT_MAX;
+    bool need[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : INT_MAX;
-      vstat = hook_batch<8>(v, vstat, u, ns);
+    for (int k = 0; k < 8; k++) need[k] = v > u[k];
+    if (marked) {
+      unsigned in[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) in[k] = need[k] ? slot_bit(p.gbits, G.slot0 + u[k]) : 0u;
+#pragma unroll
+      for (int k = 0; k < 8; k++) need[k] = need[k] && !in[k];
+    }
+    int sv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) sv[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (need[k]) {
+        if (vstat < 0) vstat = find_root(v, ns, p.path_mode);
+        vstat = hook_edge(vstat, u[k], sv[k], ns, p.path_mode);
+      }
     }
   }
 }
@@ -387,7 +479,7 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
     const int nsamp = G.n < kPickSamples ? G.n : kPickSamples;
     if (t < nsamp) {
       const int v = (int)(((long long)t * G.n) / nsamp);
-      r = find_root(v, G.nstat);
+      r = find_root(v, G.nstat, p.path_mode);
     }
   }
   s_root[t] = r;
@@ -451,7 +543,7 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
   bool member = false;
   if (R >= 0 && live) {
     const int s = lab_ld(&G.nstat[v]);
-    member = v == R || find_until(v, s, R, G.nstat) == R;
+    member = v == R || find_until(v, s, R, G.nstat, p.path_mode) == R;
   }
   const unsigned long long gw = __ballot(member);
   if (lane == 0) p.gbits[w] = gw;
@@ -467,15 +559,39 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
 // entries); list bases and root estimates of the 64 lanes sit beside it.
 // A vertex already marked as joined to the sampled root uses that root as
 // its estimate without walking.  Entries whose two ends are both marked are
-// skipped after one bitmap lookup.
+// skipped after one bitmap lookup; the rest are staged in LDS as (neighbour,
+// root estimate) pairs and hooked together when the stage fills and at the
+// end, so the streaming steps never wait for label chains.
 
 constexpr int kWaveWindow = 2048;
+constexpr int kWaveSurvivors = 256;  // staged (neighbour, root estimate) pairs per wavefront
+
+// Hooks the staged pairs of one wavefront, all lanes busy.
+__device__ __forceinline__ void flush_wave(const int2* surv, int cnt, int* ns, int mode) {
+  const int lane = lane_id();
+  for (int j0 = 0; j0 < cnt; j0 += 256) {
+    int u[4], vs[4], sv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int j = j0 + k * 64 + lane;
+      const int2 e = j < cnt ? surv[j] : make_int2(-1, 0);
+      u[k] = e.x;
+      vs[k] = e.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) sv[k] = u[k] >= 0 ? lab_ld(&ns[u[k]]) : 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (u[k] >= 0) (void)hook_edge(vs[k], u[k], sv[k], ns, mode);
+  }
+}
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   __shared__ unsigned char s_owner[4][kWaveWindow];
   __shared__ int s_base[4][64];
   __shared__ int s_root[4][64];
+  __shared__ int2 s_surv[4][kWaveSurvivors];
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
   const unsigned long long wbits = p.midbits[ws >> 6];
@@ -488,6 +604,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   const int lane = lane_id();
   const int wv = uniform((int)(threadIdx.x >> 6));
   unsigned char* const owner = s_owner[wv];
+  int2* const surv = s_surv[wv];
   const bool mine = (bits >> lane) & 1ull;
   const int v = (int)(ws + lane - G.slot0);
   const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
@@ -501,13 +618,14 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
     beg = G.nidx[v];
     deg = G.nidx[v + 1] - beg;
   }
-  if (mine && !marked) root = find_from(v, root, ns);
+  if (mine && !marked) root = find_from(v, root, ns, p.path_mode);
   const int incl = (int)wave_incl_sum((unsigned)deg);
   const int off = incl - deg;
   const int total = __shfl(incl, 63);
   s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
   s_root[wv][lane] = root;
   const int vbase = (int)(ws - G.slot0);
+  int nsurv = 0;  // wave-uniform
   for (int w0 = 0; w0 < total; w0 += kWaveWindow) {
     const int wend = w0 + kWaveWindow < total ? w0 + kWaveWindow : total;
     for (unsigned long long m = bits; m; m &= m - 1) {
@@ -530,28 +648,43 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
         src[k] = valid ? owner[e - w0] : 0;
         u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
       }
-      bool need[4];
+      unsigned keep = 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) need[k] = vbase + src[k] > u[k];
+      for (int k = 0; k < 4; k++) keep |= (vbase + src[k] > u[k]) ? (1u << k) : 0u;
       if (gw) {
-        unsigned in[4];
+        unsigned in = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          in[k] = (need[k] && ((gw >> src[k]) & 1ull)) ? slot_bit(p.gbits, G.slot0 + u[k]) : 0u;
-#pragma unroll
-        for (int k = 0; k < 4; k++) need[k] = need[k] && !in[k];
+          if (((keep >> k) & 1u) && ((gw >> src[k]) & 1ull)) in |= slot_bit(p.gbits, G.slot0 + u[k]) << k;
+        keep &= ~in;
       }
-      int sv[4];
+      // stage the entries that still need a hook; flush when the stage is full
+      const unsigned c = __popc(keep);
+      const unsigned pos = wave_incl_sum(c);
+      const int wtot = (int)__shfl(pos, 63);
+      if (wtot) {
+        if (nsurv + wtot > kWaveSurvivors) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          flush_wave(surv, nsurv, ns, p.path_mode);
+          nsurv = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        int at = nsurv + (int)(pos - c);
 #pragma unroll
-      for (int k = 0; k < 4; k++) sv[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (need[k]) (void)hook_edge(s_root[wv][src[k]], u[k], sv[k], ns);
+        for (int k = 0; k < 4; k++)
+          if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
+        nsurv += wtot;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------
@@ -567,7 +700,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
 // flight.  So the streaming part of a task never waits for label chains.
 
 template <int R, int BT>
-__device__ __forceinline__ void flush_stage(const int2* stage, int cnt, int* ns) {
+__device__ __forceinline__ void flush_stage(const int2* stage, int cnt, int* ns, int mode) {
   for (int j0 = 0; j0 < cnt; j0 += R * BT) {
     int w[R], vs[R], sv[R];
 #pragma unroll
@@ -581,7 +714,7 @@ __device__ __forceinline__ void flush_stage(const int2* stage, int cnt, int* ns)
     for (int r = 0; r < R; r++) sv[r] = w[r] >= 0 ? lab_ld(&ns[w[r]]) : 0;
 #pragma unroll
     for (int r = 0; r < R; r++)
-      if (w[r] >= 0) (void)hook_edge(vs[r], w[r], sv[r], ns);
+      if (w[r] >= 0) (void)hook_edge(vs[r], w[r], sv[r], ns, mode);
   }
 }
 
@@ -618,7 +751,7 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
     if (G.nstat != ns) {  // staged pairs belong to the previous graph: hook them first
       __syncthreads();
       const int cnt = s_cnt;
-      if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns);
+      if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns, p.path_mode);
       __syncthreads();
       if (tid == 0) s_cnt = 0;
       ns = G.nstat;
@@ -629,12 +762,12 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
     const int cbeg = (int)cb;
     const int cend = (int)(cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end);
     // root estimate of v: the sampled root if v is marked, else found once
-    const int vstat = vin ? p.giant[G.g] : find_root(v, G.nstat);
+    const int vstat = vin ? p.giant[G.g] : find_root(v, G.nstat, p.path_mode);
     for (int base = cbeg; base < cend; base += R * BT) {
       __syncthreads();
       if (s_cnt > CAP - R * BT) {
         const int cnt = s_cnt;
-        flush_stage<R, BT>(stage, cnt, ns);
+        flush_stage<R, BT>(stage, cnt, ns, p.path_mode);
         __syncthreads();
         if (tid == 0) s_cnt = 0;
         __syncthreads();
@@ -667,7 +800,7 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
   }
   __syncthreads();
   const int cnt = s_cnt;
-  if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns);
+  if (cnt > 0) flush_stage<R, BT>(stage, cnt, ns, p.path_mode);
 }
 
 // ---- flatten (ECL-CC-atomic.cu:250-264) ---------------------------------------
@@ -706,7 +839,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_init<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
-  k_hook_thread<BATCH><<<grid, block, 0, s>>>(p);
+  k_sample<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
   k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
   mark_ev();
@@ -715,6 +848,8 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
   mark_ev();
   k_mark<BATCH><<<grid, block, 0, s>>>(p);
+  mark_ev();
+  k_hook_thread<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
   k_hook_wave<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();

@@ -35,7 +35,8 @@ class EclccParams(C.Structure):
         ("hub_blocks", i32),
         ("block_threads", i32),
         ("giant_filter", i32),
-        ("reserved", i32 * 2),
+        ("path_mode", i32),
+        ("reserved", i32 * 1),
     ]
 
     def as_dict(self) -> dict:

@@ -24,6 +24,7 @@ SPACE: Dict[str, List[int]] = {
     "hub_chunk": [2048, 4096, 8192],
     "block_threads": [256, 512],
     "giant_filter": [1, 0],
+    "path_mode": [1, 0],
 }
 
 TABLE = N.PKG_ROOT / "tuning" / "mi355x.json"

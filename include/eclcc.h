@@ -60,10 +60,14 @@ typedef struct eclcc_params {
   int32_t hub_chunk;      /* edges per block task for degree > wave_max_deg */
   int32_t hub_blocks;     /* blocks of the block-level kernel (0 = as many as fit at once) */
   int32_t block_threads;  /* threads per block of the block-level kernel (256..1024) */
-  int32_t giant_filter;   /* 1: after compute1, mark the vertices joined to each graph's
-                             largest sampled tree and skip edges with both ends marked
-                             (exact; saves label gathers on power-law graphs) */
-  int32_t reserved[2];
+  int32_t giant_filter;   /* 1: after the sampled hooks, mark the vertices joined to each
+                             graph's largest tree and skip edges with both ends marked
+                             (exact; saves most label gathers) */
+  int32_t path_mode;      /* shortening of find paths: 0 = re-point every visited vertex
+                             to its grandparent (the reference's intermediate pointer
+                             jumping, ECL-CC-atomic.cu:92-104); 1 = re-point only the
+                             start vertex, to the root (one store per walk) */
+  int32_t reserved[1];
 } eclcc_params;
 
 /* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */
@@ -95,9 +99,9 @@ int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                      const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
                      const eclcc_params* params, void* stream);
 
-/* Kernels of one run, in launch order: init, compute1, hubscan, hubtop,
-   pick, mark, compute2, compute3, flatten. */
-#define ECLCC_NUM_KERNELS 9
+/* Kernels of one run, in launch order: init, sample, hubscan, hubtop, pick,
+   mark, compute1, compute2, compute3, flatten. */
+#define ECLCC_NUM_KERNELS 10
 const char* eclcc_kernel_name(int k);
 
 /* As eclcc_run_device, but records HIP events around each kernel, waits,

@@ -79,4 +79,5 @@ def test_label_accesses_go_through_atomic_helpers():
         line = body[body.rfind("\n", 0, m.start()) + 1:body.find("\n", m.end())]
         plain.append(line.strip())
     # the one plain store is init's, where no other thread reads labels
-    assert plain == ["if (live) G.nstat[v] = m;"], plain
+    assert plain == ["G.nstat[v] = m;  // no other thread reads labels during init"] or \
+        plain == ["G.nstat[v] = m;"], plain

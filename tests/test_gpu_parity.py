@@ -64,6 +64,8 @@ PARAMS = [
     dict(thread_max_deg=1 << 20, wave_max_deg=1 << 20),  # all at lane granularity
     dict(giant_filter=0),
     dict(giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=128),
+    dict(path_mode=0),
+    dict(path_mode=0, giant_filter=0, thread_max_deg=8, wave_max_deg=100),
 ]
 
 
@@ -105,7 +107,7 @@ def test_device_runs_are_repeatable(cuda):
             check(g, lab)
         assert np.array_equal(lab, first)
     kms, tot = dg.run_profiled()
-    assert len(kms) == 9 and tot > 0
+    assert len(kms) == 10 and tot > 0
     assert np.array_equal(dg.labels(), first)
 
 
@@ -131,10 +133,18 @@ def test_full_size_rmat24(cuda):
     assert hubs > 0 and tasks >= hubs
 
 
-def test_filter_on_and_off_agree_full_rmat20(cuda):
+def test_filter_and_path_modes_agree_full_rmat20(cuda):
     g = workloads.rmat(20, 16, 31)
-    announce(f"filter on/off {g.name}")
-    on = compute(g, default_params(giant_filter=1))
-    off = compute(g, default_params(giant_filter=0))
-    assert np.array_equal(on, off)
-    check(g, on)
+    announce(f"filter on/off, path modes {g.name}")
+    ref = compute(g, default_params(giant_filter=1))
+    check(g, ref)
+    for kw in (dict(giant_filter=0), dict(path_mode=0), dict(path_mode=0, giant_filter=0)):
+        assert np.array_equal(compute(g, default_params(**kw)), ref), kw
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_path_modes_on_grid_and_er(cuda, mode):
+    for g in (workloads.grid3d(48, 0.6, 4), workloads.er65536(2)):
+        announce(f"path_mode={mode} {g.name}")
+        labels, _ = compute_timed(g, default_params(path_mode=mode))
+        check(g, labels)
