@@ -561,6 +561,33 @@ int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph) {
   return check_hip(hipGraphLaunch(b->exec, s), ECLCC_ERR_LAUNCH, "graph launch failed");
 }
 
+int eclcc_batch_run_profiled(eclcc_batch* b, void* stream, float* kernel_ms, float* total_ms) {
+  if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
+  hipEvent_t ev[kNumKernels + 1] = {};
+  int rc = ECLCC_OK;
+  for (int i = 0; i <= kNumKernels && !rc; i++)
+    rc = check_hip(hipEventCreate(&ev[i]), ECLCC_ERR_DEVICE, "event create failed");
+  hipStream_t s = (hipStream_t)stream;
+  if (!rc)
+    rc = check_hip(launch_pipeline(b->params(), true, Shape{b->tuning.hub_blocks, b->tuning.block_threads},
+                                   s, ev),
+                   ECLCC_ERR_LAUNCH, "kernel launch failed");
+  if (!rc) rc = check_hip(hipEventSynchronize(ev[kNumKernels]), ECLCC_ERR_LAUNCH, "kernel execution failed");
+  if (!rc) {
+    for (int k = 0; k < kNumKernels; k++) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      if (kernel_ms) kernel_ms[k] = ms;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, ev[0], ev[kNumKernels]);
+    if (total_ms) *total_ms = tot;
+  }
+  for (int i = 0; i <= kNumKernels; i++)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  return rc;
+}
+
 int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   if (!b || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
   const GraphDesc& d = b->desc[g];

@@ -127,6 +127,7 @@ def cc_lib() -> C.CDLL:
         L.eclcc_batch_run.argtypes = [vp, vp, C.c_int]
         L.eclcc_batch_run_single.argtypes = [vp, i32, vp]
         L.eclcc_batch_set_params.argtypes = [vp, PP]
+        L.eclcc_batch_run_profiled.argtypes = [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.eclcc_batch_labels.argtypes = [vp, i32, vp]
         L.eclcc_batch_device_labels.argtypes = [vp, i32]
         L.eclcc_batch_device_labels.restype = vp

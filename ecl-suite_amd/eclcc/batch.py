@@ -47,6 +47,16 @@ class Batch:
         s = None if stream is None else stream.cuda_stream
         N.check_cc(N.cc_lib().eclcc_batch_run_single(self.handle, int(g), s))
 
+    def run_profiled(self, stream=None):
+        """Per-kernel milliseconds (eclcc.cc.KERNELS order) and the total."""
+        from .cc import KERNELS
+
+        k = (C.c_float * len(KERNELS))()
+        tot = C.c_float(0.0)
+        s = None if stream is None else stream.cuda_stream
+        N.check_cc(N.cc_lib().eclcc_batch_run_profiled(self.handle, s, k, C.byref(tot)))
+        return dict(zip(KERNELS, list(k))), tot.value
+
     def set_params(self, params: N.EclccParams) -> None:
         N.check_cc(N.cc_lib().eclcc_batch_set_params(self.handle, C.byref(params)))
         self.params = params

@@ -79,6 +79,7 @@ def batched(args) -> dict:
     end.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / args.steps
+    kms, _ = b.run_profiled(stream)
     lab_batch = b.all_labels()
 
     # one graph per launch, synchronising after each (the plain API's usage)
@@ -126,6 +127,7 @@ def batched(args) -> dict:
         "speedup_vs_plain_async": (count / (ms * 1e-3)) / (count / plain_async_s),
         "labels_batched_equal_plain": bool(same),
         "autotune_trials": len(trials),
+        "kernels_ms": kms,
     }
     b.close()
     return out

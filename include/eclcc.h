@@ -162,6 +162,10 @@ size_t eclcc_batch_device_bytes(const eclcc_batch* b);
 
 void eclcc_batch_free(eclcc_batch* b);
 
+/* As eclcc_batch_run without the hipGraph, recording HIP events around each
+   kernel; waits and writes kernel_ms[ECLCC_NUM_KERNELS] and *total_ms. */
+int eclcc_batch_run_profiled(eclcc_batch* b, void* stream, float* kernel_ms, float* total_ms);
+
 /* Runs graph g of a batch ALONE through the plain device path (its own
    kernel sequence) on `stream`: the one-graph-per-launch baseline
    the batched run is compared with, on the same resident data. */
