@@ -66,7 +66,9 @@ typedef struct eclcc_params {
   int32_t path_mode;      /* shortening of find paths: 0 = re-point every visited vertex
                              to its grandparent (the reference's intermediate pointer
                              jumping, ECL-CC-atomic.cu:92-104); 1 = re-point only the
-                             start vertex, to the root (one store per walk) */
+                             start vertex, to the root (one store per walk).  The
+                             sampled hooks walk both paths together to where they meet
+                             in either mode */
   int32_t reserved[1];
 } eclcc_params;
 
