@@ -369,24 +369,97 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
 // and walking both paths to their roots (find_from + hook_edge) is slower
 // still (21.6 ms).
 
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_sample(KParams p) {
+// One graph: one lane per vertex.
+__global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
   const long long w = ws >> 6;
   const unsigned long long act = p.lowbits[w] | p.midbits[w] | p.hubbits[w];
   const int lane = lane_id();
   if (!((act >> lane) & 1ull)) return;
-  const GraphRef G = resolve<BATCH>(p, ws);
-  const long long slot = ws + lane;
-  const int v = (int)(slot - G.slot0);
-  int* const ns = G.nstat;
-  const int u = p.second[slot];
+  int* const ns = p.nstat;
+  const int v = (int)(ws + lane);
+  const int u = p.second[v];
   const int s0 = lab_ld(&ns[v]);
   if (u < v) {
-    const int su = lab_ld(&ns[u]);
-    const int top = meet_link(s0, su, ns);
+    const int top = meet_link(s0, lab_ld(&ns[u]), ns);
     if (top < s0) lab_st(&ns[v], top);
+  }
+}
+
+// Batches: lanes are refilled.  A wavefront owns one 256-slot chunk (one
+// graph, looked up once) and a lane whose walk has ended takes the chunk's
+// next unclaimed vertex, so the wavefront does not wait for its longest path
+// with most lanes idle; every lane has one label load or CAS in flight per
+// step.  Measured on a 1024-graph batch of the ER / R-MAT mix: 13.8 -> 9.7 ms.
+// On single large graphs of uniform path length (3-D grid, ER) the extra
+// bookkeeping per step costs more than the idle lanes (grid 256^3: 0.67 ->
+// 0.75 ms), and a small graph has too few chunks to fill the GPU, hence the
+// per-lane kernel there.
+__global__ __launch_bounds__(256) void k_sample_refill(KParams p) {
+  constexpr int kItems = kSlotAlign;
+  const long long chunk = (long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6));
+  const long long ws0 = chunk * kItems;
+  if (ws0 >= p.nslots) return;
+  unsigned long long act[kItems / 64];
+#pragma unroll
+  for (int i = 0; i < kItems / 64; i++) {
+    const long long w = (ws0 >> 6) + i;
+    act[i] = p.lowbits[w] | p.midbits[w] | p.hubbits[w];
+  }
+  const GraphRef G = resolve<true>(p, ws0);
+  int* const ns = G.nstat;
+  int v = 0, s0 = 0, a = 0, b = 0;
+  bool busy = false;
+  int cursor = 0;  // wave-uniform: chunk items below it are claimed
+  for (;;) {
+    const unsigned long long idle = __ballot(!busy);
+    if (cursor < kItems) {
+      if (!busy) {
+        const int it = cursor + lanes_below(idle);
+        unsigned long long word = 0;
+#pragma unroll
+        for (int i = 0; i < kItems / 64; i++) word = (it >> 6) == i ? act[i] : word;
+        if (it < kItems && ((word >> (it & 63)) & 1ull)) {
+          const long long slot = ws0 + it;
+          v = (int)(slot - G.slot0);
+          const int u = p.second[slot];
+          s0 = lab_ld(&ns[v]);
+          if (u < v) {
+            a = s0;
+            b = lab_ld(&ns[u]);
+            busy = true;
+          }
+        }
+      }
+      cursor += __popcll(idle);
+    } else if (__ballot(busy) == 0ull) {
+      break;
+    }
+    if (busy) {
+      // one step of meet_link
+      bool done = a == b;
+      if (!done) {
+        if (a < b) {
+          const int t = a;
+          a = b;
+          b = t;
+        }
+        const int na = lab_ld(&ns[a]);
+        if (na != a) {
+          a = na;
+        } else {
+          const int seen = lab_cas(&ns[a], a, b);
+          if (seen == a) done = true;
+          else a = seen;
+        }
+        done = done || a == b;
+      }
+      if (done) {
+        if (b < s0) lab_st(&ns[v], b);
+        busy = false;
+      }
+    }
   }
 }
 
@@ -861,6 +934,7 @@ template <bool BATCH>
 static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s, hipEvent_t* ev) {
   const long long vb = (p.nslots + 255) / 256;
   if (vb <= 0) return hipSuccess;
+  if (BATCH && p.nslots % kSlotAlign) return hipErrorInvalidValue;
   const long long words = (p.nslots + 63) / 64;
   const long long sb = (words + kScanBlock - 1) / kScanBlock;
   const dim3 grid((unsigned)vb), block(256);
@@ -872,7 +946,12 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_init<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
-  k_sample<BATCH><<<grid, block, 0, s>>>(p);
+  if constexpr (BATCH) {
+    // batch slot counts are whole chunks: one wavefront per chunk
+    k_sample_refill<<<dim3((unsigned)((p.nslots / kSlotAlign + 3) / 4)), block, 0, s>>>(p);
+  } else {
+    k_sample_lane<<<grid, block, 0, s>>>(p);
+  }
   mark_ev();
   k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
   mark_ev();
