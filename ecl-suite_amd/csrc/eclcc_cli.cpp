@@ -15,6 +15,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../include/eclcc.h"
@@ -27,18 +28,23 @@ static void die(const char* msg) {
 
 int main(int argc, char* argv[]) {
   std::printf("ECL-CC (MI355X race-free build; %s)\n", eclcc_version());
-  if (argc != 2) {
-    std::fprintf(stderr, "USAGE: %s input_file_name\n\n", argv[0]);
+  // optional --check before the file: test the CSR preconditions first
+  const bool check = argc == 3 && std::strcmp(argv[1], "--check") == 0;
+  if (argc != 2 && !check) {
+    std::fprintf(stderr, "USAGE: %s [--check] input_file_name\n\n", argv[0]);
     std::exit(-1);
   }
+  const char* path = argv[argc - 1];
   eclgraph_csr* g = nullptr;
-  if (eclgraph_read(argv[1], &g) != 0) die(eclgraph_last_error());
+  if (eclgraph_read(path, &g) != 0) die(eclgraph_last_error());
   const int32_t n = eclgraph_nodes(g);
   const int32_t m = (int32_t)eclgraph_edges(g);
   const int32_t* nidx = eclgraph_nindex(g);
   const int32_t* nlist = eclgraph_nlist(g);
+  if (check && eclgraph_check(n, m, nidx, nlist, ECLGRAPH_CHECK_SYMMETRY, nullptr) != 0)
+    die(eclgraph_last_error());
 
-  std::printf("input graph: %d nodes and %d edges (%s)\n", n, m, argv[1]);
+  std::printf("input graph: %d nodes and %d edges (%s)\n", n, m, path);
   std::printf("average degree: %.2f edges per node\n", 1.0 * m / n);
   int mindeg = n, maxdeg = 0;
   for (int32_t v = 0; v < n; v++) {

@@ -312,6 +312,62 @@ int eclgraph_write(const eclgraph_csr* g, const char* path) {
   return ECLGRAPH_OK;
 }
 
+int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const int32_t* nlist,
+                   int flags, int64_t* where) {
+  if (where) *where = -1;
+  if (!nindex || (edges > 0 && !nlist)) return fail(ECLGRAPH_ERR_ARG, "null argument");
+  if (nodes < 1 || edges < 0) return fail(ECLGRAPH_ERR_ARG, "node or edge count too low");
+  auto bad = [&](int64_t at, const char* msg) {
+    if (where) *where = at;
+    return fail(ECLGRAPH_ERR_FORMAT, msg);
+  };
+  if (nindex[0] != 0) return bad(0, "nindex[0] is not 0");
+  for (int32_t v = 0; v < nodes; v++)
+    if (nindex[v + 1] < nindex[v]) return bad(v + 1, "nindex decreases");
+  if (nindex[nodes] != edges) return bad(nodes, "nindex[nodes] differs from the edge count");
+  // first offending entry, found in parallel: the minimum position over chunks
+  std::atomic<int64_t> first_range{INT64_MAX};
+  parallel_for(edges, 1 << 16, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++)
+      if (nlist[i] < 0 || nlist[i] >= nodes) {
+        int64_t cur = first_range.load();
+        while (i < cur && !first_range.compare_exchange_weak(cur, i)) {
+        }
+        break;
+      }
+  });
+  if (first_range.load() != INT64_MAX) return bad(first_range.load(), "neighbour ID out of range");
+  if (flags & ECLGRAPH_CHECK_SYMMETRY) {
+    // sorted copy of every list, then a binary search for each reverse entry
+    auto sorted = alloc_array<int32_t>(edges);
+    if (edges && !sorted) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (sorted lists)");
+    parallel_for(nodes, 1 << 12, [&](int64_t lo, int64_t hi) {
+      for (int64_t v = lo; v < hi; v++) {
+        std::copy(nlist + nindex[v], nlist + nindex[v + 1], sorted.get() + nindex[v]);
+        std::sort(sorted.get() + nindex[v], sorted.get() + nindex[v + 1]);
+      }
+    });
+    std::atomic<int64_t> first_asym{INT64_MAX};
+    parallel_for(nodes, 1 << 12, [&](int64_t lo, int64_t hi) {
+      for (int64_t v = lo; v < hi; v++)
+        for (int64_t i = nindex[v]; i < nindex[v + 1]; i++) {
+          const int32_t u = nlist[i];
+          const int32_t* b = sorted.get() + nindex[u];
+          const int32_t* e = sorted.get() + nindex[u + 1];
+          if (!std::binary_search(b, e, (int32_t)v)) {
+            int64_t cur = first_asym.load();
+            while (i < cur && !first_asym.compare_exchange_weak(cur, i)) {
+            }
+            return;
+          }
+        }
+    });
+    if (first_asym.load() != INT64_MAX)
+      return bad(first_asym.load(), "entry without its reverse (graph is not symmetric)");
+  }
+  return ECLGRAPH_OK;
+}
+
 int eclgraph_from_arrays(int32_t nodes, int64_t edges, const int32_t* nindex,
                          const int32_t* nlist, const int32_t* eweight, eclgraph_csr** out) {
   if (!out || !nindex || (edges > 0 && !nlist)) return fail(ECLGRAPH_ERR_ARG, "null argument");

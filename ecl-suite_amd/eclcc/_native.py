@@ -81,6 +81,7 @@ def graph_lib() -> C.CDLL:
         L.eclgraph_free.restype = None
         L.eclgraph_from_arrays.argtypes = [i32, i64, vp, vp, vp, C.POINTER(H)]
         L.eclgraph_from_edges.argtypes = [i32, i64, vp, vp, C.c_int, C.POINTER(H)]
+        L.eclgraph_check.argtypes = [i32, i64, vp, vp, C.c_int, C.POINTER(i64)]
         L.eclgraph_read_mtx.argtypes = [C.c_char_p, C.c_int, C.POINTER(H)]
         L.eclgraph_nodes.argtypes = [H]
         L.eclgraph_nodes.restype = i32

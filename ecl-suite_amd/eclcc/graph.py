@@ -15,6 +15,15 @@ import numpy as np
 from . import _native as N
 
 SYMMETRIZE, DROP_SELF, DEDUPE = 1, 2, 4
+CHECK_SYMMETRY = 1
+
+
+class GraphFormatError(ValueError):
+    """The CSR breaks a precondition; `where` is the offending position."""
+
+    def __init__(self, msg: str, where: int):
+        super().__init__(msg)
+        self.where = where
 
 
 class _CsrOwner:
@@ -60,6 +69,21 @@ class Graph:
         """Compulsory bytes of one CC pass: read nindex and nlist once,
         write each label once: 4(n+1) + 4m + 4n."""
         return 4 * (self.nodes + 1) + 4 * self.edges + 4 * self.nodes
+
+    def check(self, symmetry: bool = True) -> None:
+        """Raises GraphFormatError if the CSR breaks a precondition of the CC
+        path (index order, IDs in range, and with `symmetry` every entry's
+        reverse present; ECL-CC-atomic.cu:132 hooks each edge only from its
+        larger end)."""
+        ni = np.ascontiguousarray(self.nindex, dtype=np.int32)
+        nl = np.ascontiguousarray(self.nlist, dtype=np.int32)
+        where = C.c_int64(-1)
+        st = N.graph_lib().eclgraph_check(self.nodes, self.edges, ni.ctypes.data,
+                                          nl.ctypes.data if self.edges else None,
+                                          CHECK_SYMMETRY if symmetry else 0, C.byref(where))
+        if st != 0:
+            msg = N.graph_lib().eclgraph_last_error().decode()
+            raise GraphFormatError(f"{msg} (position {where.value})", where.value)
 
     # -- construction through libeclgraph.so ---------------------------------
 

@@ -33,6 +33,20 @@ enum {
   ECLGRAPH_ERR_RANGE = -5    /* a count does not fit the int32 format */
 };
 
+/* Flags of eclgraph_check. */
+#define ECLGRAPH_CHECK_SYMMETRY 1
+
+/* Checks the preconditions of the CC path that the reference leaves
+   unchecked (ECL-CC-atomic.cu:132 hooks each edge only from its larger end,
+   so an entry whose reverse is missing may be lost): nindex[0] == 0,
+   nindex non-decreasing, nindex[nodes] == edges, every nlist entry in
+   [0, nodes), and with ECLGRAPH_CHECK_SYMMETRY every entry v->u has an entry
+   u->v.  Returns 0, or ECLGRAPH_ERR_FORMAT with *where (if not NULL) = the
+   first offending nindex position (index problems) or nlist position
+   (entries), and a message. */
+int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const int32_t* nlist,
+                   int flags, int64_t* where);
+
 /* Host CSR owned by the library; arrays are read through the accessors. */
 typedef struct eclgraph_csr eclgraph_csr;
 

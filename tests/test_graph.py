@@ -157,3 +157,49 @@ def test_from_edges_flags():
     h = Graph.from_edges(4, [0, 1, 1, 2, 3], [1, 0, 1, 2, 2])
     assert_clean_symmetric(h)
     assert h.edges == 4
+
+
+def test_check_accepts_generated_and_flags_broken_csr():
+    from eclcc.graph import GraphFormatError
+
+    for g in (workloads.er65536(), workloads.rmat(10, 8, seed=3), workloads.grid3d(8, 0.5, 2)):
+        g.check()
+    # the reverse of entry 2->5 removed: reported at that entry's position
+    g = Graph.from_edges(8, [0, 1, 2, 3], [1, 2, 5, 4])
+    src = np.repeat(np.arange(g.nodes), np.diff(g.nindex))
+    drop = int(np.flatnonzero((src == 5) & (g.nlist == 2))[0])
+    keep = np.ones(g.edges, dtype=bool)
+    keep[drop] = False
+    ni = g.nindex.copy()
+    ni[6:] -= 1
+    asym = Graph(ni, g.nlist[keep])
+    with pytest.raises(GraphFormatError) as e:
+        asym.check()
+    pos = int(np.flatnonzero((np.repeat(np.arange(8), np.diff(ni)) == 2) & (asym.nlist == 5))[0])
+    assert e.value.where == pos and "symmetric" in str(e.value)
+    asym.check(symmetry=False)  # index and range are fine
+    bad = Graph(g.nindex.copy(), g.nlist.copy())
+    bad.nlist[3] = 8
+    with pytest.raises(GraphFormatError) as e:
+        bad.check(symmetry=False)
+    assert e.value.where == 3
+    bad = Graph(g.nindex.copy(), g.nlist.copy())
+    bad.nindex[4] = bad.nindex[5] + 1
+    with pytest.raises(GraphFormatError) as e:
+        bad.check()
+    assert e.value.where == 5 and "nindex decreases" in str(e.value)
+
+
+def test_cli_check_flag(tmp_path):
+    from eclcc import _native as N
+
+    exe = N.BIN_DIR / "eclcc"
+    g = Graph.from_edges(4, [0, 2], [1, 3])
+    ok = tmp_path / "ok.egr"
+    g.write(str(ok))
+    asym = tmp_path / "asym.egr"
+    Graph(np.array([0, 1, 1, 1, 1], dtype=np.int32), np.array([1], dtype=np.int32)).write(str(asym))
+    r = subprocess.run([str(exe), "--bogus", str(ok)], capture_output=True, text=True)
+    assert r.returncode == 255 and "USAGE:" in r.stderr
+    r = subprocess.run([str(exe), "--check", str(asym)], capture_output=True, text=True)
+    assert r.returncode == 255 and "not symmetric" in r.stderr
