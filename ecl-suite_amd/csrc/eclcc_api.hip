@@ -471,13 +471,16 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
   (void)hipGetDevice(&b->device);
   b->ngraphs = ngraphs;
   b->desc.resize(ngraphs);
+  // packed layout: graph g's index at ibase[g], entries at ebase[g] and
+  // labels at its first slot, all in one allocation each
+  std::vector<long long> ibase((size_t)ngraphs), ebase((size_t)ngraphs);
   long long vb = 0, ib = 0, eb = 0, hub_cap = 0, max_n = 0, max_m = 0;
   for (int32_t g = 0; g < ngraphs; g++) {
     if ((rc = check_csr_args(nodes[g], edges[g], nindex[g], nlist[g]))) return rc;
     GraphDesc& d = b->desc[g];
     d.vbase = vb;
-    d.ibase = ib;
-    d.ebase = eb;
+    ibase[g] = ib;
+    ebase[g] = eb;
     d.n = nodes[g];
     d.m = edges[g];
     vb += ((long long)nodes[g] + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
@@ -501,6 +504,12 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
   b->work1_bytes = workspace_bytes(max_n, hub_capacity(max_m, b->tuning.wave_max, 1), 1);
   if ((rc = batch_alloc(b.get(), &b->d_work1, b->work1_bytes, "single-graph workspace"))) return rc;
 
+  for (int32_t g = 0; g < ngraphs; g++) {
+    GraphDesc& d = b->desc[g];
+    d.nidx = b->d_nidx + ibase[g];
+    d.nlist = b->d_nlist + ebase[g];
+    d.nstat = b->d_nstat + d.vbase;
+  }
   std::vector<int32_t> chunk((size_t)nchunks);
   for (int32_t g = 0; g < ngraphs; g++) {
     const long long c0 = b->desc[g].vbase / kSlotAlign;
@@ -516,11 +525,11 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
     return rc;
   for (int32_t g = 0; g < ngraphs; g++) {
     const GraphDesc& d = b->desc[g];
-    if ((rc = check_hip(hipMemcpy(b->d_nidx + d.ibase, nindex[g], sizeof(int32_t) * ((size_t)d.n + 1), h2d),
+    if ((rc = check_hip(hipMemcpy((void*)d.nidx, nindex[g], sizeof(int32_t) * ((size_t)d.n + 1), h2d),
                         ECLCC_ERR_COPY, "copying to device failed")))
       return rc;
     if (d.m > 0 &&
-        (rc = check_hip(hipMemcpy(b->d_nlist + d.ebase, nlist[g], sizeof(int32_t) * (size_t)d.m, h2d),
+        (rc = check_hip(hipMemcpy((void*)d.nlist, nlist[g], sizeof(int32_t) * (size_t)d.m, h2d),
                         ECLCC_ERR_COPY, "copying to device failed")))
       return rc;
   }
@@ -600,8 +609,8 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   prm.block_threads = b->tuning.block_threads;
   prm.giant_filter = b->tuning.filter;
   prm.path_mode = b->tuning.path_mode;
-  return eclcc_run_device(d.n, d.m, b->d_nidx + d.ibase, b->d_nlist + d.ebase,
-                          b->d_nstat + d.vbase, b->d_work1, b->work1_bytes, &prm, stream);
+  return eclcc_run_device(d.n, d.m, d.nidx, d.nlist, d.nstat, b->d_work1, b->work1_bytes, &prm,
+                          stream);
 }
 
 int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) {
@@ -650,14 +659,14 @@ int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) {
 int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat) {
   if (!b || !nstat || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
   const GraphDesc& d = b->desc[g];
-  return check_hip(hipMemcpy(nstat, b->d_nstat + d.vbase, sizeof(int32_t) * (size_t)d.n,
+  return check_hip(hipMemcpy(nstat, d.nstat, sizeof(int32_t) * (size_t)d.n,
                              hipMemcpyDeviceToHost),
                    ECLCC_ERR_COPY, "copying from device failed");
 }
 
 int32_t* eclcc_batch_device_labels(eclcc_batch* b, int32_t g) {
   if (!b || g < 0 || g >= b->ngraphs) return nullptr;
-  return b->d_nstat + b->desc[g].vbase;
+  return b->desc[g].nstat;
 }
 
 size_t eclcc_batch_device_bytes(const eclcc_batch* b) { return b ? b->device_bytes : 0; }

@@ -24,13 +24,19 @@ constexpr int kScanBlock = 1024;
 constexpr int kPickSamples = 256;
 constexpr int kNumKernels = ECLCC_NUM_KERNELS;
 
-// One graph inside packed batch arrays.
+// One graph of a batch.  Its arrays may sit anywhere in device memory (packed
+// into one allocation by eclcc_batch, placed in the server's arena, or owned
+// by the caller for device-resident requests); 64-bit pointers, so a batch
+// may hold any total of vertices and entries.  Its vertices occupy the slots
+// vbase .. vbase + n - 1 of the batch's slot space, which indexes the
+// workspace bit arrays.
 struct GraphDesc {
-  long long vbase;  // first label slot (multiple of kSlotAlign)
-  long long ibase;  // first entry of its nindex (n+1 local offsets)
-  long long ebase;  // first entry of its nlist
-  int n;            // vertices
-  int m;            // CSR entries
+  const int* nidx;   // n+1 index entries (offsets into its nlist)
+  const int* nlist;  // m adjacency entries
+  int* nstat;        // n labels
+  long long vbase;   // first slot (multiple of kSlotAlign)
+  int n;             // vertices
+  int m;             // CSR entries
 };
 
 // High-degree task list entry: vertex slot, index of its first edge task and
@@ -44,9 +50,9 @@ struct HubEntry {
 
 // Everything the kernels read; passed by value.
 struct KParams {
-  const int* nidx;   // single: nindex; batch: packed nindex
-  const int* nlist;  // single: nlist; batch: packed nlist
-  int* nstat;        // single: labels; batch: packed label slots
+  const int* nidx;   // single graph: nindex (batches: per graph in desc)
+  const int* nlist;  // single graph: nlist
+  int* nstat;        // single graph: labels
   int n;             // single: vertex count (unused for batches)
   int ngraphs;       // 1 for a single graph
   long long nslots;  // slots covered by the vertex-parallel kernels

@@ -101,7 +101,7 @@ __device__ __forceinline__ GraphRef graph_ref(const KParams& p, int g) {
     return GraphRef{p.nidx, p.nlist, p.nstat, p.n, 0, 0};
   } else {
     const GraphDesc d = p.desc[g];
-    return GraphRef{p.nidx + d.ibase, p.nlist + d.ebase, p.nstat + d.vbase, d.n, g, d.vbase};
+    return GraphRef{d.nidx, d.nlist, d.nstat, d.n, g, d.vbase};
   }
 }
 

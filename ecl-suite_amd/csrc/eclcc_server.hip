@@ -92,6 +92,7 @@ class Arena {
 struct Request {
   int64_t id;
   int32_t n, m;
+  bool resident;           // true: nindex, nlist and out are device pointers
   const int32_t* nindex;
   const int32_t* nlist;
   int32_t* out;
@@ -104,10 +105,13 @@ struct Request {
   size_t stage_off = 0;  // byte offset of its labels in the pinned arena
 };
 
-size_t request_device_bytes(int32_t n, int32_t m) {
+// Arena bytes of a request besides the batch workspace: its CSR and labels
+// when they come from the host, and its graph-table and chunk-map entries.
+size_t request_device_bytes(int32_t n, int32_t m, bool resident) {
   const size_t slots = ((size_t)n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
-  return align256(4 * ((size_t)n + 1)) + align256(4 * (size_t)m) + align256(4 * slots) +
-         align256(sizeof(GraphDesc)) + align256(4 * (slots / kSlotAlign) + 4);
+  const size_t tables = align256(sizeof(GraphDesc)) + align256(4 * (slots / kSlotAlign) + 4);
+  if (resident) return tables;
+  return align256(4 * ((size_t)n + 1)) + align256(4 * (size_t)m) + align256(4 * (size_t)n) + tables;
 }
 
 struct Batch {
@@ -162,18 +166,14 @@ struct eclcc_server {
     // layout inside the batch's arena block
     std::vector<GraphDesc>& desc = b->desc;
     desc.resize(G);
-    long long vb = 0, ib = 0, eb = 0, hub_cap = 0;
+    long long vb = 0, hub_cap = 0;
     for (int g = 0; g < G; g++) {
       Request* r = b->reqs[g];
       desc[g].vbase = vb;
-      desc[g].ibase = ib;
-      desc[g].ebase = eb;
       desc[g].n = r->n;
       desc[g].m = r->m;
       r->slot0 = vb;
       vb += ((long long)r->n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
-      ib += (long long)r->n + 1;
-      eb += r->m;
       hub_cap += hub_capacity(r->m, tuning.wave_max, 1);
     }
     const long long nchunks = vb / kSlotAlign;
@@ -193,9 +193,19 @@ struct eclcc_server {
     };
     GraphDesc* d_desc = (GraphDesc*)carve(sizeof(GraphDesc) * G);
     int32_t* d_chunk = (int32_t*)carve(4 * (size_t)nchunks);
-    int32_t* d_nidx = (int32_t*)carve(4 * (size_t)ib);
-    int32_t* d_nlist = (int32_t*)carve(4 * (size_t)eb);
-    int32_t* d_nstat = (int32_t*)carve(4 * (size_t)vb);
+    // host requests: CSR and labels in the arena; resident ones: the caller's arrays
+    for (int g = 0; g < G; g++) {
+      Request* r = b->reqs[g];
+      if (r->resident) {
+        desc[g].nidx = r->nindex;
+        desc[g].nlist = r->nlist;
+        desc[g].nstat = r->out;
+      } else {
+        desc[g].nidx = (const int*)carve(4 * ((size_t)r->n + 1));
+        desc[g].nlist = (const int*)carve(4 * (size_t)r->m);
+        desc[g].nstat = (int*)carve(4 * (size_t)r->n);
+      }
+    }
     const size_t wbytes = workspace_bytes(vb, hub_cap, G);
     void* d_work = carve(wbytes);
     if (off > b->dev_bytes) return set_error(ECLCC_ERR_STATE, "server batch layout overflow");
@@ -210,17 +220,15 @@ struct eclcc_server {
                      ECLCC_ERR_COPY, "copying to device failed");
     for (int g = 0; g < G && !rc; g++) {
       Request* r = b->reqs[g];
-      rc = check_hip(hipMemcpyAsync(d_nidx + desc[g].ibase, r->nindex, 4 * ((size_t)r->n + 1), h2d, s),
+      if (r->resident) continue;
+      rc = check_hip(hipMemcpyAsync((void*)desc[g].nidx, r->nindex, 4 * ((size_t)r->n + 1), h2d, s),
                      ECLCC_ERR_COPY, "copying to device failed");
       if (!rc && r->m > 0)
-        rc = check_hip(hipMemcpyAsync(d_nlist + desc[g].ebase, r->nlist, 4 * (size_t)r->m, h2d, s),
+        rc = check_hip(hipMemcpyAsync((void*)desc[g].nlist, r->nlist, 4 * (size_t)r->m, h2d, s),
                        ECLCC_ERR_COPY, "copying to device failed");
     }
     if (rc) return rc;
     KParams p{};
-    p.nidx = d_nidx;
-    p.nlist = d_nlist;
-    p.nstat = d_nstat;
     p.ngraphs = G;
     p.nslots = vb;
     p.desc = d_desc;
@@ -230,7 +238,8 @@ struct eclcc_server {
                    ECLCC_ERR_LAUNCH, "kernel launch failed");
     for (int g = 0; g < G && !rc; g++) {
       Request* r = b->reqs[g];
-      rc = check_hip(hipMemcpyAsync(h_stage + r->stage_off, d_nstat + r->slot0, 4 * (size_t)r->n,
+      if (r->resident) continue;
+      rc = check_hip(hipMemcpyAsync(h_stage + r->stage_off, desc[g].nstat, 4 * (size_t)r->n,
                                     hipMemcpyDeviceToHost, s),
                      ECLCC_ERR_COPY, "copying from device failed");
     }
@@ -242,7 +251,8 @@ struct eclcc_server {
     const auto now = Clock::now();
     std::lock_guard<std::mutex> lock(mu);
     for (Request* r : b->reqs) {
-      if (status == ECLCC_OK) std::memcpy(r->out, h_stage + r->stage_off, 4 * (size_t)r->n);
+      if (status == ECLCC_OK && !r->resident)
+        std::memcpy(r->out, h_stage + r->stage_off, 4 * (size_t)r->n);
       r->status = status;
       r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
       r->done = true;
@@ -291,9 +301,9 @@ struct eclcc_server {
             Request* r = pending.front();
             const long long vb2 = vb + ((long long)r->n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
             const long long cap2 = hub_cap + hub_capacity(r->m, tuning.wave_max, 1);
-            const size_t req2 = req_bytes + request_device_bytes(r->n, r->m);
+            const size_t req2 = req_bytes + request_device_bytes(r->n, r->m, r->resident);
             const size_t need_d = 4096 + req2 + workspace_bytes(vb2, cap2, (long long)nb->reqs.size() + 1);
-            const size_t need_s = sbytes + align256(4 * (size_t)r->n);
+            const size_t need_s = sbytes + (r->resident ? 0 : align256(4 * (size_t)r->n));
             if (dev.used() + need_d > dev.capacity() || stage.used() + need_s > stage.capacity())
               break;
             r->stage_off = sbytes;
@@ -396,20 +406,25 @@ int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* para
   return ECLCC_OK;
 }
 
-int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
-                            const int32_t* nlist, int32_t* labels_out) {
+}  // extern "C"
+
+namespace {
+
+int64_t submit_request(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
+                       const int32_t* nlist, int32_t* labels_out, bool resident) {
   if (!s) return set_error(ECLCC_ERR_ARG, "null server");
   if (nodes < 1 || edges < 0 || !nindex || (edges > 0 && !nlist) || !labels_out)
     return set_error(ECLCC_ERR_ARG, "bad request arguments");
-  const size_t need = 4096 + request_device_bytes(nodes, edges) +
+  const size_t need = 4096 + request_device_bytes(nodes, edges, resident) +
                       workspace_bytes(((long long)nodes + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
                                       hub_capacity(edges, s->tuning.wave_max, 1), 1);
-  if (need > s->dev.capacity() || align256(4 * (size_t)nodes) > s->stage.capacity())
+  if (need > s->dev.capacity() || (!resident && align256(4 * (size_t)nodes) > s->stage.capacity()))
     return set_error(ECLCC_ERR_BUDGET, "request larger than the HBM budget");
   Request* r = new (std::nothrow) Request;
   if (!r) return set_error(ECLCC_ERR_ALLOC, "host allocation failed");
   r->n = nodes;
   r->m = edges;
+  r->resident = resident;
   r->nindex = nindex;
   r->nlist = nlist;
   r->out = labels_out;
@@ -426,6 +441,21 @@ int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const
   }
   s->cv_work.notify_one();
   return r->id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
+                            const int32_t* nlist, int32_t* labels_out) {
+  return submit_request(s, nodes, edges, nindex, nlist, labels_out, false);
+}
+
+int64_t eclcc_server_submit_device(eclcc_server* s, int32_t nodes, int32_t edges,
+                                   const int32_t* d_nindex, const int32_t* d_nlist,
+                                   int32_t* d_labels) {
+  return submit_request(s, nodes, edges, d_nindex, d_nlist, d_labels, true);
 }
 
 int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms) {

@@ -9,7 +9,7 @@ HIP library libeclcc.so; importing works without a GPU, calling needs one.
 from ._native import EclccError, EclccParams, EclgraphError, NativeLibraryMissing
 from .graph import Graph
 from .cc import DeviceGraph, compute, compute_cc, compute_timed, default_params, version
-from . import workloads
+from . import roofline, workloads
 
 __all__ = [
     "Graph", "DeviceGraph", "compute", "compute_cc", "compute_timed", "default_params",

@@ -1,0 +1,78 @@
+"""Algorithmic bytes of one CC pass and of each of its kernels.
+
+Whole pass (the compulsory traffic: read the CSR once, write each label
+once):  B(n, m) = 4(n+1) + 4m + 4n.
+
+Per kernel, the bytes its own job cannot avoid moving:
+
+    init      4(n+1) + 4n      read the index, write every label
+    sample    4 per active vertex (its second list entry)
+    hubscan, hubtop, pick      bookkeeping (no per-vertex or per-edge term)
+    mark      n/8              one bit per vertex
+    compute1  4 per adjacency entry + 8 per vertex of the low-degree bin
+    compute2  4 per adjacency entry + 8 per vertex of the medium-degree bin
+    compute3  4 per adjacency entry + 8 per vertex of the high-degree bin
+    flatten   8n               read and write every label
+
+A vertex is active when it has a smaller neighbour (only those hook,
+ECL-CC-atomic.cu:119-132); the bins split the active vertices by degree:
+low <= thread_max_deg < medium <= wave_max_deg < high.
+"""
+from __future__ import annotations
+
+from typing import Dict
+
+import numpy as np
+
+PER_UNIT = {
+    "init": "4(n+1) + 4n bytes per graph",
+    "sample": "4 bytes per active vertex",
+    "mark": "1/8 byte per vertex",
+    "compute1": "4 bytes per entry + 8 bytes per vertex of the low-degree bin",
+    "compute2": "4 bytes per entry + 8 bytes per vertex of the medium-degree bin",
+    "compute3": "4 bytes per entry + 8 bytes per vertex of the high-degree bin",
+    "flatten": "8 bytes per vertex",
+}
+
+
+def pass_bytes(n: int, m: int) -> int:
+    return 4 * (n + 1) + 4 * m + 4 * n
+
+
+def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) -> Dict[str, tuple]:
+    """(vertices, entries) of the active vertices in each degree bin."""
+    n = nindex.shape[0] - 1
+    deg = np.diff(nindex).astype(np.int64)
+    has = deg > 0
+    smallest = np.full(n, np.iinfo(np.int32).max, dtype=np.int64)
+    if nlist.shape[0]:
+        starts = nindex[:-1][has]
+        smallest[has] = np.minimum.reduceat(nlist, starts)
+    active = smallest < np.arange(n)
+    low = active & (deg <= thread_max)
+    mid = active & (deg > thread_max) & (deg <= wave_max)
+    hub = active & (deg > wave_max)
+    return {
+        "active": (int(active.sum()), int(deg[active].sum())),
+        "low": (int(low.sum()), int(deg[low].sum())),
+        "mid": (int(mid.sum()), int(deg[mid].sum())),
+        "hub": (int(hub.sum()), int(deg[hub].sum())),
+    }
+
+
+def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) -> Dict[str, int]:
+    n = nindex.shape[0] - 1
+    b = bins(nindex, nlist, thread_max, wave_max)
+    per_bin = {k: 4 * e + 8 * v for k, (v, e) in b.items()}
+    return {
+        "init": 4 * (n + 1) + 4 * n,
+        "sample": 4 * b["active"][0],
+        "hubscan": 0,
+        "hubtop": 0,
+        "pick": 0,
+        "mark": (n + 7) // 8,
+        "compute1": per_bin["low"],
+        "compute2": per_bin["mid"],
+        "compute3": per_bin["hub"],
+        "flatten": 8 * n,
+    }

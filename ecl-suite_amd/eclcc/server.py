@@ -1,8 +1,10 @@
 """Continuous-batching CC server (``eclcc_server_*`` in libeclcc.so).
 
-Requests are host CSR arrays; the native worker admits them by HBM budget,
-batches whatever has arrived into one launch sequence per stream, overlaps
-batches on several streams and copies the labels back.
+Requests are host CSR arrays (copied in, labels copied back) or graphs that
+already live in device memory (labels written in place, nothing crosses
+PCIe).  The native worker admits them by HBM budget, batches whatever has
+arrived into one launch sequence per stream and overlaps batches on several
+streams.
 """
 from __future__ import annotations
 
@@ -29,7 +31,7 @@ class Server:
                                                   C.byref(h)))
         self._h = h
         # request id -> (graph arrays kept alive, output array)
-        self._live: Dict[int, Tuple[np.ndarray, np.ndarray, np.ndarray]] = {}
+        self._live: Dict[int, Tuple] = {}
 
     def submit(self, g: Graph) -> int:
         ni = np.ascontiguousarray(g.nindex, dtype=np.int32)
@@ -39,6 +41,24 @@ class Server:
                                              nl.ctypes.data if g.edges else None, out.ctypes.data)
         N.check_cc(int(rid))
         self._live[int(rid)] = (ni, nl, out)
+        return int(rid)
+
+    def submit_device(self, nodes: int, edges: int, nindex, nlist, labels) -> int:
+        """Request on a device-resident graph: `nindex`, `nlist` and `labels`
+        are contiguous int32 tensors on the server's device (labels gets
+        `nodes` entries).  They are kept alive until the request is waited
+        for; wait() then returns `labels`."""
+        for t, need in ((nindex, nodes + 1), (nlist, edges), (labels, nodes)):
+            if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch.int32" \
+                    or t.numel() < need:
+                raise ValueError("nindex/nlist/labels must be contiguous int32 device tensors "
+                                 "of at least nodes+1 / edges / nodes entries")
+        rid = N.cc_lib().eclcc_server_submit_device(self._h, int(nodes), int(edges),
+                                                    nindex.data_ptr(),
+                                                    nlist.data_ptr() if edges else None,
+                                                    labels.data_ptr())
+        N.check_cc(int(rid))
+        self._live[int(rid)] = (nindex, nlist, labels)
         return int(rid)
 
     def wait(self, rid: int, timeout_ms: int = -1) -> Tuple[np.ndarray, float]:

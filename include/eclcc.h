@@ -195,8 +195,18 @@ int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* para
 int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
                             const int32_t* nlist, int32_t* labels_out);
 
+/* Submits one CC request on a graph that is already in device memory (a
+   graph store in HBM, or the output of an earlier GPU stage): d_nindex,
+   d_nlist and d_labels are device pointers on the server's device and must
+   stay valid until the request completes; the labels are written to
+   d_labels[nodes] and nothing crosses PCIe.  The budget covers only the
+   batch workspace.  Returns a request id or a negative status. */
+int64_t eclcc_server_submit_device(eclcc_server* s, int32_t nodes, int32_t edges,
+                                   const int32_t* d_nindex, const int32_t* d_nlist,
+                                   int32_t* d_labels);
+
 /* Waits for request `id` (timeout_ms < 0: forever).  *latency_ms (optional)
-   = submit -> labels in host memory. */
+   = submit -> labels in host memory (device memory for a device request). */
 int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms);
 
 /* Server counters: completed requests, launched batches, peak device bytes,

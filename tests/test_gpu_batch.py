@@ -86,3 +86,38 @@ def test_server_rejects_request_over_budget(cuda):
         with pytest.raises(EclccError) as ei:
             s.submit(g)
         assert ei.value.status == -6
+
+
+def test_server_device_resident_requests(cuda):
+    import torch
+
+    gs = small_mix(30, seed=93, n_hi=60000)
+    announce(f"server with {len(gs)} device-resident requests mixed with host requests")
+    dev = torch.device("cuda:0")
+    res = []
+    for g in gs:
+        ni = torch.from_numpy(np.ascontiguousarray(g.nindex)).to(dev)
+        nl = torch.from_numpy(np.ascontiguousarray(g.nlist)).to(dev) if g.edges else \
+            torch.zeros(1, dtype=torch.int32, device=dev)
+        lab = torch.full((g.nodes,), -1, dtype=torch.int32, device=dev)
+        res.append((ni, nl, lab))
+    torch.cuda.synchronize()
+    with Server(hbm_budget_bytes=128 << 20, streams=2, max_batch=32) as s:
+        ids = []
+        for i, (g, (ni, nl, lab)) in enumerate(zip(gs, res)):
+            ids.append(("dev", s.submit_device(g.nodes, g.edges, ni, nl, lab)))
+            if i % 3 == 0:
+                ids.append(("host", s.submit(g)))
+        k = 0
+        for kind, rid in ids:
+            out, lat = s.wait(rid, timeout_ms=60000)
+            got = out.cpu().numpy() if kind == "dev" else out
+            g = gs[k] if kind == "dev" else gs[k - 1]
+            if kind == "dev":
+                k += 1
+            assert lat > 0
+            assert np.array_equal(got, oracle.cc_labels(g.nindex, g.nlist)), (kind, g.name)
+        assert s.stats()["max_batch"] > 1
+    with pytest.raises(ValueError):
+        with Server(hbm_budget_bytes=64 << 20, streams=1) as s:
+            s.submit_device(gs[0].nodes, gs[0].edges, res[0][0].cpu(), res[0][1], res[0][2])

@@ -203,3 +203,30 @@ def test_cli_check_flag(tmp_path):
     assert r.returncode == 255 and "USAGE:" in r.stderr
     r = subprocess.run([str(exe), "--check", str(asym)], capture_output=True, text=True)
     assert r.returncode == 255 and "not symmetric" in r.stderr
+
+
+def test_kernel_bytes_partition_the_active_entries():
+    from eclcc import roofline
+
+    g = workloads.rmat(12, 16, seed=3)
+    b = roofline.bins(g.nindex, g.nlist, 16, 2048)
+    # hand count with a plain loop
+    lo = mid = hub = 0
+    ent = [0, 0, 0]
+    for v in range(g.nodes):
+        lst = g.nlist[g.nindex[v]:g.nindex[v + 1]]
+        if len(lst) == 0 or lst.min() >= v:
+            continue
+        d = len(lst)
+        k = 0 if d <= 16 else (1 if d <= 2048 else 2)
+        ent[k] += d
+        lo, mid, hub = lo + (k == 0), mid + (k == 1), hub + (k == 2)
+    assert b["low"] == (lo, ent[0]) and b["mid"] == (mid, ent[1]) and b["hub"] == (hub, ent[2])
+    assert b["active"] == (lo + mid + hub, sum(ent))
+    kb = roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048)
+    assert kb["compute2"] == 4 * ent[1] + 8 * mid
+    assert roofline.pass_bytes(g.nodes, g.edges) == g.algorithmic_bytes()
+    # a star with its hub last: one active high-degree vertex
+    s = Graph.from_edges(3000, np.arange(2999), np.full(2999, 2999))
+    b = roofline.bins(s.nindex, s.nlist, 16, 2048)
+    assert b["hub"] == (1, 2999) and b["low"] == (0, 0)
