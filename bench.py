@@ -217,7 +217,7 @@ def main() -> None:
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
     ap.add_argument("--serve-streams", type=int, default=4)
-    ap.add_argument("--serve-max-batch", type=int, default=256)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -120,6 +120,7 @@ struct Batch {
   std::vector<int32_t> chunk;
   size_t dev_off = 0, dev_bytes = 0;
   size_t stage_off = 0, stage_bytes = 0;
+  size_t table_off = 0;  // graph table and chunk map, pinned, after the labels
   int stream = -1;
   hipEvent_t done_ev = nullptr;
 };
@@ -213,10 +214,16 @@ struct eclcc_server {
     if (!carve_workspace(d_work, wbytes, vb, hub_cap, G, &w))
       return set_error(ECLCC_ERR_STATE, "server workspace carve failed");
     const hipMemcpyKind h2d = hipMemcpyHostToDevice;
-    int rc = check_hip(hipMemcpyAsync(d_desc, desc.data(), sizeof(GraphDesc) * G, h2d, s),
+    // the tables go up from pinned memory: an asynchronous copy from
+    // pageable memory would be staged by the runtime on the host
+    char* h_tab = h_stage + b->table_off;
+    std::memcpy(h_tab, desc.data(), sizeof(GraphDesc) * G);
+    std::memcpy(h_tab + align256(sizeof(GraphDesc) * G), chunk.data(), 4 * (size_t)nchunks);
+    int rc = check_hip(hipMemcpyAsync(d_desc, h_tab, sizeof(GraphDesc) * G, h2d, s),
                        ECLCC_ERR_COPY, "copying to device failed");
     if (!rc)
-      rc = check_hip(hipMemcpyAsync(d_chunk, chunk.data(), 4 * (size_t)nchunks, h2d, s),
+      rc = check_hip(hipMemcpyAsync(d_chunk, h_tab + align256(sizeof(GraphDesc) * G),
+                                    4 * (size_t)nchunks, h2d, s),
                      ECLCC_ERR_COPY, "copying to device failed");
     for (int g = 0; g < G && !rc; g++) {
       Request* r = b->reqs[g];
@@ -294,7 +301,7 @@ struct eclcc_server {
         if (stop && pending.empty() && !busy) return;
         if (free_k >= 0 && !pending.empty()) {
           nb = new Batch;
-          size_t sbytes = 0, req_bytes = 0, exact_d = 0;
+          size_t sbytes = 0, req_bytes = 0, exact_d = 0, tab_bytes = 0;
           long long vb = 0, hub_cap = 0;
           while (!pending.empty() && (int)nb->reqs.size() < cfg.max_batch &&
                  inflight + (int64_t)nb->reqs.size() < cfg.max_inflight) {
@@ -304,10 +311,14 @@ struct eclcc_server {
             const size_t req2 = req_bytes + request_device_bytes(r->n, r->m, r->resident);
             const size_t need_d = 4096 + req2 + workspace_bytes(vb2, cap2, (long long)nb->reqs.size() + 1);
             const size_t need_s = sbytes + (r->resident ? 0 : align256(4 * (size_t)r->n));
-            if (dev.used() + need_d > dev.capacity() || stage.used() + need_s > stage.capacity())
+            const size_t tab2 = align256(sizeof(GraphDesc) * (nb->reqs.size() + 1)) +
+                                align256(4 * (size_t)(vb2 / kSlotAlign));
+            if (dev.used() + need_d > dev.capacity() ||
+                stage.used() + need_s + tab2 > stage.capacity())
               break;
             r->stage_off = sbytes;
             sbytes = need_s;
+            tab_bytes = tab2;
             vb = vb2;
             hub_cap = cap2;
             req_bytes = req2;
@@ -320,7 +331,7 @@ struct eclcc_server {
             nb = nullptr;
           } else {
             const bool ok = dev.alloc(exact_d, &nb->dev_off);
-            const bool ok2 = ok && stage.alloc(sbytes, &nb->stage_off);
+            const bool ok2 = ok && stage.alloc(sbytes + tab_bytes, &nb->stage_off);
             if (!ok2) {
               if (ok) dev.release(nb->dev_off, exact_d);
               // fragmented: put the requests back and wait for a batch to retire
@@ -329,7 +340,8 @@ struct eclcc_server {
               nb = nullptr;
             } else {
               nb->dev_bytes = exact_d;
-              nb->stage_bytes = sbytes;
+              nb->stage_bytes = sbytes + tab_bytes;
+              nb->table_off = nb->stage_off + sbytes;
               for (Request* r : nb->reqs) r->stage_off += nb->stage_off;
               inflight += (int64_t)nb->reqs.size();
               batches++;
