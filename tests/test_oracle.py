@@ -82,11 +82,10 @@ def test_bfs_and_unionfind_agree_on_random_graphs():
 
 
 def test_brute_force_tiny():
-    # every graph on 5 vertices with a given edge subset: labels by closure
-    rng = np.random.default_rng(1)
+    # every graph on 5 vertices (all 1024 edge subsets): labels by closure
     pairs = [(i, j) for i in range(5) for j in range(i + 1, 5)]
-    for _ in range(60):
-        keep = [p for p in pairs if rng.random() < 0.3]
+    for mask in range(1 << len(pairs)):
+        keep = [p for k, p in enumerate(pairs) if (mask >> k) & 1]
         g = G.from_undirected(5, keep if keep else np.zeros((0, 2)))
         reach = np.eye(5, dtype=bool)
         for a, b in keep:
@@ -94,7 +93,11 @@ def test_brute_force_tiny():
         for k in range(5):  # Warshall closure
             reach |= reach[:, [k]] & reach[[k], :]
         expect = np.array([np.nonzero(reach[v])[0].min() for v in range(5)], dtype=np.int32)
-        assert np.array_equal(oracle.cc_labels(*csr(g)), expect)
+        ni, nl = csr(g)
+        assert np.array_equal(oracle.cc_labels(ni, nl), expect), keep
+        assert np.array_equal(oracle.cc_bfs(ni, nl), expect), keep
+        assert np.array_equal(oracle.eclcc_serial(ni, nl), expect), keep
+        assert oracle.verify(ni, nl, expect) == (0, len(set(expect.tolist())))
 
 
 def plant(path):
