@@ -115,8 +115,9 @@ def plain(args) -> dict:
     pass_ms = statistics.mean(tots)
     hubs, tasks = dg.hub_stats()
     # roofline of the kernel that takes the most time, on its own algorithmic bytes
-    kb = roofline.kernel_bytes(g.nindex, g.nlist, dg.params.thread_max_deg, dg.params.wave_max_deg)
-    dom = max(KERNELS, key=lambda k: kernel_ms[k])
+    kb = roofline.kernel_bytes(g.nindex, g.nlist, dg.params.thread_max_deg, dg.params.wave_max_deg,
+                               dg.params.giant_filter)
+    dom = max((k for k in KERNELS if kb[k]), key=lambda k: kernel_ms[k])
     dom_gbs = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -166,12 +167,44 @@ def plain(args) -> dict:
     tr = REPO / "profiles" / f"{args.workload}_traffic.json"
     if tr.exists():
         t = json.loads(tr.read_text())
-        if t.get("edges") == g.edges and t.get("nodes") == g.nodes:
+        if (t.get("edges") == g.edges and t.get("nodes") == g.nodes
+                and t.get("params", dg.params.as_dict()) == dg.params.as_dict()):
             kt = t.get("kernels", {}).get(dom)
             if kt is not None:
                 out["roofline"]["traffic"] = kt["fetch_bytes"] + kt["write_bytes"]
             out["pass_roofline"]["traffic"] = t["bytes_per_pass"]
             out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+    # the same graph with every adjacency list read (giant_filter 1)
+    if dg.params.giant_filter == 2 and not args.no_compare:
+        del dg
+        torch.cuda.empty_cache()
+        d1 = DeviceGraph(g, "cuda:0", default_params(giant_filter=1))
+        for _ in range(args.warmup):
+            d1.run(stream)
+        per1 = [[] for _ in KERNELS]
+        for _ in range(max(3, min(args.steps, 10))):
+            kms1, _ = d1.run_profiled(stream)
+            for i, x in enumerate(kms1):
+                per1[i].append(x)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            d1.run(stream)
+        end.record(stream)
+        torch.cuda.synchronize()
+        ms1 = start.elapsed_time(end) / args.steps
+        same1 = bool((d1.labels() == labels).all())
+        out["read_every_edge"] = {
+            "params": d1.params.as_dict(),
+            "ms_per_step": ms1,
+            "edges_per_s": g.edges / (ms1 * 1e-3),
+            "pct_hbm_peak": 100.0 * B / (ms1 * 1e-3) / HBM_PEAK,
+            "kernels_ms": {k: statistics.mean(v) for k, v in zip(KERNELS, per1)},
+            "labels_equal": same1,
+        }
+        if not same1:
+            raise SystemExit("labels differ between giant_filter 2 and 1")
+        del d1
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, args.cpu_budget, labels)
@@ -206,6 +239,8 @@ def main() -> None:
     ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served", "copy"])
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the read-every-edge (giant_filter 1) comparison pass")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     # batched / served
     ap.add_argument("--batch-graphs", type=int, default=4096)

@@ -59,7 +59,9 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   t.hub_chunk = p.hub_chunk;
   t.block_threads = p.block_threads;
   t.hub_blocks = p.hub_blocks;
-  t.filter = p.giant_filter ? 1 : 0;
+  if (p.giant_filter < 0 || p.giant_filter > 2)
+    return set_error(ECLCC_ERR_ARG, "giant_filter must be 0, 1 or 2");
+  t.filter = p.giant_filter;
   if (p.path_mode != 0 && p.path_mode != 1) return set_error(ECLCC_ERR_ARG, "path_mode must be 0 or 1");
   t.path_mode = p.path_mode;
   if (t.thread_max < 0 || t.wave_max < t.thread_max)
@@ -88,7 +90,7 @@ size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs) {
   const size_t words = (size_t)((nslots + 63) / 64);
   const size_t blocks = (words + kScanBlock - 1) / kScanBlock;
   size_t b = 256;                                                   // counters, flags
-  b += 5 * align_up(sizeof(unsigned long long) * words, 256);  // low, mid, hub, info, member words
+  b += 6 * align_up(sizeof(unsigned long long) * words, 256);  // act, low, mid, hub, info, member words
   b += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
   b += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
   b += align_up(sizeof(int) * (size_t)std::max<long long>(nslots, 1), 256);  // second entries
@@ -106,8 +108,8 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
   w->hubctr = reinterpret_cast<unsigned long long*>(p);
   w->flags = reinterpret_cast<unsigned int*>(p + 128);
   p += 256;
-  unsigned long long** wordarrays[5] = {&w->lowbits, &w->midbits, &w->hubbits, &w->hubinfo,
-                                        &w->gbits};
+  unsigned long long** wordarrays[6] = {&w->actbits, &w->lowbits, &w->midbits, &w->hubbits,
+                                        &w->hubinfo,  &w->gbits};
   for (auto a : wordarrays) {
     *a = reinterpret_cast<unsigned long long*>(p);
     p += align_up(sizeof(unsigned long long) * words, 256);
@@ -124,6 +126,7 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
 }
 
 void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
+  p->actbits = w.actbits;
   p->lowbits = w.lowbits;
   p->midbits = w.midbits;
   p->hubbits = w.hubbits;
@@ -227,8 +230,8 @@ const char* eclcc_status_string(int s) {
 const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64)"; }
 
 const char* eclcc_kernel_name(int k) {
-  static const char* names[kNumKernels] = {"init",     "sample",   "hubscan",  "hubtop",
-                                           "pick",     "mark",     "compute1", "compute2",
+  static const char* names[kNumKernels] = {"init",     "sample",   "pick",     "mark",
+                                           "hubscan",  "hubtop",   "compute1", "compute2",
                                            "compute3", "flatten"};
   return (k >= 0 && k < kNumKernels) ? names[k] : "";
 }
@@ -241,7 +244,7 @@ void eclcc_params_default(eclcc_params* p) {
   p->hub_chunk = 4096;
   p->hub_blocks = 0;
   p->block_threads = 256;
-  p->giant_filter = 1;
+  p->giant_filter = 2;
   p->path_mode = 1;
 }
 

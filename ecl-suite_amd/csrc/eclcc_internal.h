@@ -59,10 +59,11 @@ struct KParams {
   const GraphDesc* desc;   // batch: graph table
   const int* chunk_graph;  // batch: graph of each 256-slot chunk
   // one 64-bit word per 64 slots
-  unsigned long long* lowbits;   // low-degree vertices with a smaller neighbour (init -> compute1)
-  unsigned long long* midbits;   // medium-degree vertices (init -> compute2)
-  unsigned long long* hubbits;   // high-degree vertices (init -> mark)
-  unsigned long long* hubinfo;   // hubs | tasks << 32 of the word, then its prefix in its block
+  unsigned long long* actbits;   // vertices with a smaller neighbour (init)
+  unsigned long long* lowbits;   // low-degree vertices to process (init, final after mark -> compute1)
+  unsigned long long* midbits;   // medium-degree vertices to process (-> compute2)
+  unsigned long long* hubbits;   // high-degree vertices to process (-> compute1 emits their tasks)
+  unsigned long long* hubinfo;   // hubs | tasks << 32 of the word (mark), then its prefix in its block
   unsigned long long* gbits;     // members of the sampled largest component (mark -> compute2/3)
   unsigned long long* blocktot;  // per kScanBlock words: totals, then exclusive prefix
   int* giant;                    // per graph: sampled root of the largest component, or -1
@@ -74,7 +75,8 @@ struct KParams {
   int thread_max;
   int wave_max;
   int hub_chunk;
-  int filter;     // 1: skip edges with both ends in the sampled largest component
+  int filter;     // 0: off; 1: skip edges with both ends in the sampled largest
+                  // component; 2: also leave the lists of its members unread
   int path_mode;  // 0: pointer jumping at every step (reference); 1: root jump
 };
 
@@ -82,6 +84,7 @@ struct KParams {
 struct Workspace {
   unsigned long long* hubctr;
   unsigned int* flags;
+  unsigned long long* actbits;
   unsigned long long* lowbits;
   unsigned long long* midbits;
   unsigned long long* hubbits;

@@ -13,12 +13,12 @@
 //   init      labels := first smaller neighbour (reference init); degree
 //             binning into 64-bit ballot words (no atomics)
 //   sample    hooks of each vertex's second list entry
+//   pick      per graph: the most frequent root among 256 sampled vertices
+//   mark      bitmap of the vertices already joined to that root; final work
+//             lists; high-degree counts per word
 //   hubscan   exclusive prefix of the high-degree counts per 1024-word block
 //   hubtop    prefix over the blocks; total hubs and edge tasks
-//   pick      per graph: the most frequent root among 256 sampled vertices
-//   mark      bitmap of the vertices already joined to that root; writes the
-//             high-degree task list in slot order
-//   compute1  low degree: lane granularity
+//   compute1  writes the high-degree task list; low degree: lane granularity
 //   compute2  medium degree: wavefront granularity
 //   compute3  high degree: block granularity, edge tasks staged through LDS
 //   flatten   labels := root (reference flatten)
@@ -28,8 +28,12 @@
 // from a 2 MB table that stays in L2 at ~265 G/s.  After init and the
 // sampled hooks almost every remaining edge of random, grid and power-law
 // graphs joins two vertices that are already in the largest tree; one bit
-// per vertex (2 MB for 2^24) tells that without touching the labels.  Skipping an edge whose ends are both
-// connected to the sampled root is exact: they are in one component already.
+// per vertex (2 MB for 2^24) tells that without touching the labels.
+// Skipping an edge whose ends are both connected to the sampled root is
+// exact: they are in one component already.  Filter 1 reads every entry and
+// skips such edges after a lookup; filter 2 does not read the lists of the
+// tree's members at all: every edge with an end outside the tree is hooked
+// from that end, which hooks all of its entries.
 
 #include <climits>
 
@@ -256,19 +260,6 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
   return hook(vstat, r, nstat);
 }
 
-// Hooks the neighbours u[0..K) of v that are smaller than v; all K label
-// loads are in flight before the first one is used.
-template <int K>
-__device__ __forceinline__ int hook_batch(int v, int vstat, const int (&u)[K], int* ns, int mode) {
-  int s[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) s[k] = v > u[k] ? lab_ld(&ns[u[k]]) : 0;
-#pragma unroll
-  for (int k = 0; k < K; k++)
-    if (v > u[k]) vstat = hook_edge(vstat, u[k], s[k], ns, mode);
-  return vstat;
-}
-
 // ---- init (ECL-CC-atomic.cu:69-88) and degree binning ------------------------
 // label[v] = first neighbour smaller than v in list order, else v.  Each lane
 // checks the first four entries of its list; lists that need a longer scan
@@ -326,28 +317,25 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
     G.nstat[v] = m;  // no other thread reads labels during init
     p.second[slot] = x1;
   }
-  // degree bins of the vertices that have a smaller neighbour
+  // degree bins: of the vertices that have a smaller neighbour (they hook
+  // their entries below them), and with filter 2 of every vertex that has
+  // a neighbour (mark then keeps those outside the largest tree)
   const bool active = live && m != v;
-  const int deg = active ? end - beg : 0;
-  const bool low = active && deg <= p.thread_max;
+  const bool binned = p.filter == 2 ? live && end > beg : active;
+  const int deg = binned ? end - beg : 0;
+  const bool low = binned && deg <= p.thread_max;
   const bool mid = deg > p.thread_max && deg <= p.wave_max;
   const bool hub = deg > p.wave_max;
+  const unsigned long long actmask = __ballot(active);
   const unsigned long long lowmask = __ballot(low);
   const unsigned long long midmask = __ballot(mid);
   const unsigned long long hubmask = __ballot(hub);
-  unsigned long long info = 0;
-  if (hubmask) {
-    unsigned tasks = hub ? (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk) : 0u;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
-    info = (unsigned long long)__popcll(hubmask) | ((unsigned long long)tasks << 32);
-  }
   if (lane == 0) {
     const long long w = ws >> 6;
+    p.actbits[w] = actmask;
     p.lowbits[w] = lowmask;
     p.midbits[w] = midmask;
     p.hubbits[w] = hubmask;
-    p.hubinfo[w] = info;
   }
   if (slot == 0) {
     *p.hubctr = 0ull;
@@ -373,8 +361,7 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
 __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const long long w = ws >> 6;
-  const unsigned long long act = p.lowbits[w] | p.midbits[w] | p.hubbits[w];
+  const unsigned long long act = p.actbits[ws >> 6];
   const int lane = lane_id();
   if (!((act >> lane) & 1ull)) return;
   int* const ns = p.nstat;
@@ -404,8 +391,7 @@ __global__ __launch_bounds__(256) void k_sample_refill(KParams p) {
   unsigned long long act[kItems / 64];
 #pragma unroll
   for (int i = 0; i < kItems / 64; i++) {
-    const long long w = (ws0 >> 6) + i;
-    act[i] = p.lowbits[w] | p.midbits[w] | p.hubbits[w];
+    act[i] = p.actbits[(ws0 >> 6) + i];
   }
   const GraphRef G = resolve<true>(p, ws0);
   int* const ns = G.nstat;
@@ -464,29 +450,85 @@ __global__ __launch_bounds__(256) void k_sample_refill(KParams p) {
 }
 
 // ---- compute1 (rest of ECL-CC-atomic.cu:108-157) -------------------------------
-// Low degree (<= thread_max): one lane per vertex hooks all entries smaller
-// than it, eight list entries in flight per step.  A vertex marked as joined
-// to the sampled root takes that root as its estimate and skips every entry
-// whose other end is marked too, after one bitmap lookup.
+// Each wavefront first writes the high-degree task entries of its word, in
+// slot order, at the place the scan gave them.  Then low degree
+// (<= thread_max): one lane per vertex, eight list entries in flight per step.
+//  * filter 0/1: the lane hooks the entries smaller than its vertex.  A
+//    vertex marked as joined to the sampled root R takes R as its estimate
+//    and skips every entry whose other end is marked too, after one bitmap
+//    lookup.
+//  * filter 2 in a graph with R: the work list holds only vertices outside
+//    R's tree; the lane hooks every entry, and a neighbour inside the tree
+//    is replaced by R (they are joined, so the hook is the same).  An edge
+//    with both ends in the tree is hooked by nobody: its ends are already
+//    in one tree.  An edge with one end outside is hooked from that end.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const unsigned long long bits = p.lowbits[ws >> 6];
-  if (!bits) return;
+  const long long w = ws >> 6;
+  const unsigned long long hb = p.hubbits[w];
+  const unsigned long long bits = p.lowbits[w];
+  if (!hb && !bits) return;
   const GraphRef G = resolve<BATCH>(p, ws);
   const int lane = lane_id();
-  if (!((bits >> lane) & 1ull)) return;
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
+  if (hb) {
+    const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
+    const bool hub = (hb >> lane) & 1ull;
+    unsigned tasks = 0;
+    int hbeg = 0, hend = 0;
+    if (hub) {
+      hbeg = G.nidx[v];
+      hend = G.nidx[v + 1];
+      tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
+    }
+    const unsigned excl = wave_incl_sum(tasks) - tasks;
+    if (hub) {
+      const long long idx = (long long)(unsigned)base + lanes_below(hb);
+      if (idx < p.hub_cap) {
+        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl, hbeg, hend};
+      } else {
+        __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  if (!((bits >> lane) & 1ull)) return;
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
-  const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
-  const bool marked = (gw >> lane) & 1ull;
+  const int R = p.filter ? p.giant[G.g] : -1;
   const int beg = G.nidx[v];
   const int end = G.nidx[v + 1];
-  int vstat = marked ? p.giant[G.g] : -1;  // found on first use otherwise
+  if (p.filter == 2 && R >= 0) {
+    int vstat = -1;  // found on first use
+    for (int i = beg; i < end; i += 8) {
+      int u[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : -1;
+      bool need[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) need[k] = u[k] >= 0 && u[k] != v;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (need[k] && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
+      int sv[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) sv[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (need[k]) {
+          if (vstat < 0) vstat = find_root(v, ns, p.path_mode);
+          vstat = hook_edge(vstat, u[k], sv[k], ns, p.path_mode);
+        }
+      }
+    }
+    return;
+  }
+  const unsigned long long gw = p.filter ? p.gbits[w] : 0ull;
+  const bool marked = (gw >> lane) & 1ull;
+  int vstat = marked ? R : -1;  // found on first use otherwise
   for (int i = beg; i < end; i += 8) {
     int u[8];
 #pragma unroll
@@ -610,7 +652,13 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
   }
 }
 
-// ---- mark: membership bitmap and the high-degree task list -----------------
+// ---- mark: membership bitmap and the final work lists ------------------------
+// gbits := the vertices joined to the graph's sampled root R (exact: no hook
+// runs during mark).  With filter 2 the work lists keep only the vertices
+// outside R's tree (each then hooks all of its entries), or, in a graph
+// without a sampled root, the vertices that have a smaller neighbour (each
+// hooks the entries below it, as the reference does).  Also counts each
+// word's high-degree vertices to process and their edge tasks for the scan.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mark(KParams p) {
@@ -622,29 +670,6 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
-  // high-degree vertices of this word, in slot order
-  const unsigned long long hb = p.hubbits[w];
-  if (hb) {
-    const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
-    const bool hub = (hb >> lane) & 1ull;
-    unsigned tasks = 0;
-    int hbeg = 0, hend = 0;
-    if (hub) {
-      hbeg = G.nidx[v];
-      hend = G.nidx[v + 1];
-      tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
-    }
-    const unsigned excl = wave_incl_sum(tasks) - tasks;
-    if (hub) {
-      const long long idx = (long long)(unsigned)base + lanes_below(hb);
-      if (idx < p.hub_cap) {
-        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl, hbeg, hend};
-      } else {
-        __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  }
-  // vertices already joined to the graph's sampled root
   const int R = p.filter ? p.giant[G.g] : -1;
   bool member = false;
   if (R >= 0 && live) {
@@ -652,7 +677,29 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
     member = v == R || find_until(v, s, R, G.nstat, p.path_mode) == R;
   }
   const unsigned long long gw = __ballot(member);
-  if (lane == 0) p.gbits[w] = gw;
+  unsigned long long keep = ~0ull;
+  if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
+  const unsigned long long hb = p.hubbits[w] & keep;
+  unsigned long long info = 0;
+  if (hb) {
+    unsigned tasks = 0;
+    if ((hb >> lane) & 1ull) {
+      const int deg = G.nidx[v + 1] - G.nidx[v];
+      tasks = (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
+    info = (unsigned long long)__popcll(hb) | ((unsigned long long)tasks << 32);
+  }
+  if (lane == 0) {
+    p.gbits[w] = gw;
+    p.hubinfo[w] = info;
+    if (p.filter == 2) {
+      p.lowbits[w] &= keep;
+      p.midbits[w] &= keep;
+      p.hubbits[w] = hb;
+    }
+  }
 }
 
 // ---- compute2 (ECL-CC-atomic.cu:161-199) --------------------------------------
@@ -665,9 +712,11 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
 // entries); list bases and root estimates of the 64 lanes sit beside it.
 // A vertex already marked as joined to the sampled root uses that root as
 // its estimate without walking.  Entries whose two ends are both marked are
-// skipped after one bitmap lookup; the rest are staged in LDS as (neighbour,
-// root estimate) pairs and hooked together when the stage fills and at the
-// end, so the streaming steps never wait for label chains.
+// skipped after one bitmap lookup (filter 1); with filter 2 the vertices
+// here lie outside the largest tree and every entry is kept, a neighbour
+// inside the tree replaced by its root.  The kept entries are staged in LDS
+// as (neighbour, root estimate) pairs and hooked together when the stage
+// fills and at the end, so the streaming steps never wait for label chains.
 
 constexpr int kWaveWindow = 2048;
 constexpr int kWaveSurvivors = 256;  // staged (neighbour, root estimate) pairs per wavefront
@@ -713,8 +762,10 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   int2* const surv = s_surv[wv];
   const bool mine = (bits >> lane) & 1ull;
   const int v = (int)(ws + lane - G.slot0);
+  const int R = p.filter ? p.giant[G.g] : -1;
+  const bool skip = p.filter == 2 && R >= 0;  // owners lie outside R's tree
   const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
-  const int groot = gw ? p.giant[G.g] : -1;
+  const int groot = gw ? R : -1;
   // a marked vertex is joined to the sampled root: that root is a valid
   // estimate (the CAS of a hook checks it), no walk needed
   const bool marked = (gw >> lane) & 1ull;
@@ -755,9 +806,18 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
         u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
       }
       unsigned keep = 0;
+      if (skip) {
+        // every entry; a neighbour inside the largest tree stands for R
 #pragma unroll
-      for (int k = 0; k < 4; k++) keep |= (vbase + src[k] > u[k]) ? (1u << k) : 0u;
-      if (gw) {
+        for (int k = 0; k < 4; k++) keep |= (u[k] != INT_MAX && u[k] != vbase + src[k]) ? (1u << k) : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) keep |= (vbase + src[k] > u[k]) ? (1u << k) : 0u;
+      }
+      if (gw && !skip) {
         unsigned in = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -799,7 +859,8 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
 // list).  A task is hub_chunk consecutive entries of one adjacency list,
 // read in rounds of kHubPerThread coalesced loads per thread, all in flight
 // together.  The entries that still need a hook (< v, and not both ends in
-// the largest tree) are staged in LDS as (neighbour, root estimate of v)
+// the largest tree; with filter 2, every entry of a listed vertex, which
+// lies outside that tree) are staged in LDS as (neighbour, root estimate of v)
 // pairs -- one LDS atomic per wavefront per round -- and kept there across
 // tasks; the block hooks them all together when the stage is about to fill
 // and at the end, with every thread busy and kHubPerThread label loads in
@@ -863,12 +924,14 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
       ns = G.nstat;
     }
     const int v = (int)(h.slot - G.slot0);
-    const bool vin = p.filter && slot_bit(p.gbits, h.slot);
+    const int Rg = p.filter ? p.giant[G.g] : -1;
+    const bool skip = p.filter == 2 && Rg >= 0;  // v lies outside Rg's tree
+    const bool vin = !skip && p.filter && slot_bit(p.gbits, h.slot);
     const long long cb = (long long)h.beg + (t - h.first_task) * (long long)p.hub_chunk;
     const int cbeg = (int)cb;
     const int cend = (int)(cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end);
     // root estimate of v: the sampled root if v is marked, else found once
-    const int vstat = vin ? p.giant[G.g] : find_root(v, G.nstat, p.path_mode);
+    const int vstat = vin ? Rg : find_root(v, G.nstat, p.path_mode);
     for (int base = cbeg; base < cend; base += R * BT) {
       __syncthreads();
       if (s_cnt > CAP - R * BT) {
@@ -884,12 +947,21 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
         const int i = base + r * BT + tid;
         u[r] = i < cend ? ld_stream(&G.nlist[i]) : INT_MAX;
       }
-      unsigned in[R];
-#pragma unroll
-      for (int r = 0; r < R; r++) in[r] = (vin && u[r] < v) ? slot_bit(p.gbits, G.slot0 + u[r]) : 0u;
       unsigned keep = 0;
+      if (skip) {
+        // every entry; a neighbour inside the largest tree stands for its root
 #pragma unroll
-      for (int r = 0; r < R; r++) keep |= (u[r] < v && !in[r]) ? (1u << r) : 0u;
+        for (int r = 0; r < R; r++) keep |= (u[r] != INT_MAX && u[r] != v) ? (1u << r) : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (((keep >> r) & 1u) && slot_bit(p.gbits, G.slot0 + u[r])) u[r] = Rg;
+      } else {
+        unsigned in[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) in[r] = (vin && u[r] < v) ? slot_bit(p.gbits, G.slot0 + u[r]) : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++) keep |= (u[r] < v && !in[r]) ? (1u << r) : 0u;
+      }
       const unsigned c = __popc(keep);
       const unsigned incl = wave_incl_sum(c);
       const unsigned wtot = __shfl(incl, 63);
@@ -953,13 +1025,13 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
     k_sample_lane<<<grid, block, 0, s>>>(p);
   }
   mark_ev();
-  k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
-  mark_ev();
-  k_hub_top<<<dim3(1), dim3(1024), 0, s>>>(p);
-  mark_ev();
   k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
   mark_ev();
   k_mark<BATCH><<<grid, block, 0, s>>>(p);
+  mark_ev();
+  k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
+  mark_ev();
+  k_hub_top<<<dim3(1), dim3(1024), 0, s>>>(p);
   mark_ev();
   k_hook_thread<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();

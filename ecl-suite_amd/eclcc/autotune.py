@@ -23,7 +23,7 @@ SPACE: Dict[str, List[int]] = {
     "thread_max_deg": [8, 16, 32],
     "hub_chunk": [2048, 4096, 8192],
     "block_threads": [256, 512],
-    "giant_filter": [1, 0],
+    "giant_filter": [2, 1, 0],
     "path_mode": [1, 0],
 }
 

@@ -19,7 +19,7 @@ import numpy as np
 from . import _native as N
 from .graph import Graph
 
-KERNELS = ("init", "sample", "hubscan", "hubtop", "pick", "mark", "compute1", "compute2",
+KERNELS = ("init", "sample", "pick", "mark", "hubscan", "hubtop", "compute1", "compute2",
            "compute3", "flatten")
 
 

@@ -16,11 +16,14 @@ Per kernel, the bytes its own job cannot avoid moving:
 
 A vertex is active when it has a smaller neighbour (only those hook,
 ECL-CC-atomic.cu:119-132); the bins split the active vertices by degree:
-low <= thread_max_deg < medium <= wave_max_deg < high.
+low <= thread_max_deg < medium <= wave_max_deg < high.  That is the work of
+giant_filter 0 and 1.  With giant_filter 2 the bins hold only the vertices
+outside the sampled largest tree, which the device decides during the run:
+the compute kernels' figures are then not known on the host (None).
 """
 from __future__ import annotations
 
-from typing import Dict
+from typing import Dict, Optional
 
 import numpy as np
 
@@ -60,10 +63,13 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) 
     }
 
 
-def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) -> Dict[str, int]:
+def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
+                 giant_filter: int = 1) -> Dict[str, Optional[int]]:
     n = nindex.shape[0] - 1
     b = bins(nindex, nlist, thread_max, wave_max)
-    per_bin = {k: 4 * e + 8 * v for k, (v, e) in b.items()}
+    per_bin: Dict[str, Optional[int]] = {k: 4 * e + 8 * v for k, (v, e) in b.items()}
+    if giant_filter == 2:
+        per_bin.update(low=None, mid=None, hub=None)
     return {
         "init": 4 * (n + 1) + 4 * n,
         "sample": 4 * b["active"][0],

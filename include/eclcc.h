@@ -60,9 +60,13 @@ typedef struct eclcc_params {
   int32_t hub_chunk;      /* edges per block task for degree > wave_max_deg */
   int32_t hub_blocks;     /* blocks of the block-level kernel (0 = as many as fit at once) */
   int32_t block_threads;  /* threads per block of the block-level kernel (256..1024) */
-  int32_t giant_filter;   /* 1: after the sampled hooks, mark the vertices joined to each
-                             graph's largest tree and skip edges with both ends marked
-                             (exact; saves most label gathers) */
+  int32_t giant_filter;   /* after the sampled hooks, mark the vertices joined to each
+                             graph's largest tree (exact: an edge with both ends marked
+                             joins one tree already).  0: off, every edge hooked from its
+                             larger end as in the reference; 1: every list read, edges
+                             with both ends marked skipped after a bitmap lookup; 2
+                             (default): lists of marked vertices not read at all, each
+                             unmarked vertex hooks all of its entries */
   int32_t path_mode;      /* shortening of find paths: 0 = re-point every visited vertex
                              to its grandparent (the reference's intermediate pointer
                              jumping, ECL-CC-atomic.cu:92-104); 1 = re-point only the

@@ -35,6 +35,12 @@ def test_params_defaults_and_workspace():
     assert L.eclcc_workspace_bytes(0, 10, C.byref(p)) == 0
     bad = eclcc.default_params(block_threads=300)
     assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
+    assert p.giant_filter == 2 and p.path_mode == 1
+    for f in (0, 1, 2):
+        assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(giant_filter=f))) > 0
+    for bad in (eclcc.default_params(giant_filter=3), eclcc.default_params(giant_filter=-1),
+                eclcc.default_params(path_mode=2)):
+        assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
 
 
 def test_argument_errors_are_statuses():

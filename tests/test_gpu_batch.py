@@ -44,17 +44,26 @@ def test_batch_matches_oracle_and_single(cuda):
             assert np.array_equal(lab, e), g.name
 
 
-def test_batch_with_tuned_params(cuda):
+@pytest.mark.parametrize("filt", [0, 1, 2])
+def test_batch_with_tuned_params(cuda, filt):
     import torch
 
-    gs = small_mix(10, seed=5, n_hi=60000)
-    announce("batch with non-default params")
-    p = default_params(thread_max_deg=2, wave_max_deg=32, hub_chunk=256, block_threads=512)
+    gs = small_mix(10, seed=5, n_hi=60000) + [workloads.grid3d(20, 0.2, 3), workloads.rmat(12, 2, 4)]
+    announce(f"batch with non-default params, giant_filter={filt}")
+    p = default_params(thread_max_deg=2, wave_max_deg=32, hub_chunk=256, block_threads=512,
+                       giant_filter=filt)
+    expect = [oracle.cc_labels(g.nindex, g.nlist) for g in gs]
     with Batch(gs, p) as b:
+        for rep in range(2):
+            b.run(use_graph=True)
+            torch.cuda.synchronize()
+            for g, e, lab in zip(gs, expect, b.all_labels()):
+                assert np.array_equal(lab, e), (rep, g.name)
+        b.set_params(default_params(giant_filter=filt))
         b.run(use_graph=True)
         torch.cuda.synchronize()
-        for g, lab in zip(gs, b.all_labels()):
-            assert np.array_equal(lab, oracle.cc_labels(g.nindex, g.nlist)), g.name
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), g.name
 
 
 def test_server_alone_and_in_mix(cuda):

@@ -66,6 +66,11 @@ PARAMS = [
     dict(giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=128),
     dict(path_mode=0),
     dict(path_mode=0, giant_filter=0, thread_max_deg=8, wave_max_deg=100),
+    dict(giant_filter=2),
+    dict(giant_filter=2, thread_max_deg=0, wave_max_deg=1),
+    dict(giant_filter=2, thread_max_deg=2, wave_max_deg=40, hub_chunk=128, path_mode=0),
+    dict(giant_filter=2, thread_max_deg=1 << 20, wave_max_deg=1 << 20),
+    dict(giant_filter=2, thread_max_deg=16, wave_max_deg=704, block_threads=1024, hub_blocks=3),
 ]
 
 
@@ -118,19 +123,35 @@ def test_full_size_grid256(cuda):
     dg.run()
     lab = dg.labels()
     check(g, lab)
+    other = 1 if dg.params.giant_filter == 2 else 2
+    announce(f"full size {g.name} giant_filter={other}")
+    dg2 = DeviceGraph(g, "cuda:0", default_params(giant_filter=other))
+    dg2.run()
+    assert np.array_equal(dg2.labels(), lab)
+    del dg2
     # size-independent properties
     assert np.all(lab <= np.arange(g.nodes)) and np.all(lab[lab] == lab)
 
 
 def test_full_size_rmat24(cuda):
     g = workloads.rmat24()
-    announce(f"full size {g.name} n={g.nodes} m={g.edges}")
-    dg = DeviceGraph(g, "cuda:0")
-    dg.run()
-    lab = dg.labels()
-    check(g, lab)
-    hubs, tasks = dg.hub_stats()
-    assert hubs > 0 and tasks >= hubs
+    first = None
+    for filt in (1, 2):
+        announce(f"full size {g.name} n={g.nodes} m={g.edges} giant_filter={filt}")
+        dg = DeviceGraph(g, "cuda:0", default_params(giant_filter=filt))
+        dg.run()
+        lab = dg.labels()
+        hubs, tasks = dg.hub_stats()
+        if first is None:
+            check(g, lab)
+            first = lab
+            # every high-degree vertex with a smaller neighbour has tasks
+            assert hubs > 0 and tasks >= hubs
+        else:
+            assert np.array_equal(lab, first)
+            # only those outside the largest tree
+            assert tasks >= hubs >= 0
+        del dg
 
 
 def test_filter_and_path_modes_agree_full_rmat20(cuda):
@@ -138,13 +159,14 @@ def test_filter_and_path_modes_agree_full_rmat20(cuda):
     announce(f"filter on/off, path modes {g.name}")
     ref = compute(g, default_params(giant_filter=1))
     check(g, ref)
-    for kw in (dict(giant_filter=0), dict(path_mode=0), dict(path_mode=0, giant_filter=0)):
+    for kw in (dict(giant_filter=0), dict(path_mode=0), dict(path_mode=0, giant_filter=0),
+               dict(giant_filter=2), dict(giant_filter=2, path_mode=0)):
         assert np.array_equal(compute(g, default_params(**kw)), ref), kw
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_path_modes_on_grid_and_er(cuda, mode):
-    for g in (workloads.grid3d(48, 0.6, 4), workloads.er65536(2)):
-        announce(f"path_mode={mode} {g.name}")
-        labels, _ = compute_timed(g, default_params(path_mode=mode))
+@pytest.mark.parametrize("mode,filt", [(0, 1), (1, 1), (0, 2), (1, 2), (1, 0)])
+def test_path_and_filter_modes_on_grid_and_er(cuda, mode, filt):
+    for g in (workloads.grid3d(48, 0.6, 4), workloads.er65536(2), workloads.grid3d(40, 0.25, 5)):
+        announce(f"path_mode={mode} giant_filter={filt} {g.name}")
+        labels, _ = compute_timed(g, default_params(path_mode=mode, giant_filter=filt))
         check(g, labels)
