@@ -374,77 +374,50 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   }
 }
 
-// Batches: lanes are refilled.  A wavefront owns one 256-slot chunk (one
-// graph, looked up once) and a lane whose walk has ended takes the chunk's
-// next unclaimed vertex, so the wavefront does not wait for its longest path
-// with most lanes idle; every lane has one label load or CAS in flight per
-// step.  Measured on a 1024-graph batch of the ER / R-MAT mix: 13.8 -> 9.7 ms.
-// On single large graphs of uniform path length (3-D grid, ER) the extra
-// bookkeeping per step costs more than the idle lanes (grid 256^3: 0.67 ->
-// 0.75 ms), and a small graph has too few chunks to fill the GPU, hence the
-// per-lane kernel there.
-__global__ __launch_bounds__(256) void k_sample_refill(KParams p) {
-  constexpr int kItems = kSlotAlign;
-  const long long chunk = (long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6));
-  const long long ws0 = chunk * kItems;
-  if (ws0 >= p.nslots) return;
-  unsigned long long act[kItems / 64];
-#pragma unroll
-  for (int i = 0; i < kItems / 64; i++) {
-    act[i] = p.actbits[(ws0 >> 6) + i];
-  }
-  const GraphRef G = resolve<true>(p, ws0);
-  int* const ns = G.nstat;
-  int v = 0, s0 = 0, a = 0, b = 0;
-  bool busy = false;
-  int cursor = 0;  // wave-uniform: chunk items below it are claimed
-  for (;;) {
-    const unsigned long long idle = __ballot(!busy);
-    if (cursor < kItems) {
-      if (!busy) {
-        const int it = cursor + lanes_below(idle);
-        unsigned long long word = 0;
-#pragma unroll
-        for (int i = 0; i < kItems / 64; i++) word = (it >> 6) == i ? act[i] : word;
-        if (it < kItems && ((word >> (it & 63)) & 1ull)) {
-          const long long slot = ws0 + it;
-          v = (int)(slot - G.slot0);
-          const int u = p.second[slot];
-          s0 = lab_ld(&ns[v]);
-          if (u < v) {
-            a = s0;
-            b = lab_ld(&ns[u]);
-            busy = true;
-          }
-        }
-      }
-      cursor += __popcll(idle);
-    } else if (__ballot(busy) == 0ull) {
-      break;
+// Batches: a block gathers the active vertices of kSampleWords words into
+// an LDS list and its threads work through it, one meeting walk per thread,
+// so no lane idles on padding slots or inactive vertices and each thread
+// looks up its graph once per vertex.  Measured on a 1024-graph batch of the
+// ER / R-MAT mix: 9.7 ms with one lane per slot (lanes refilled within each
+// 256-slot chunk), 5.6 ms dense.  On one large graph the per-lane kernel is
+// faster (R-MAT-24: 0.20 vs 0.23 ms; 3-D grid 256^3: 0.67 vs 0.73 ms): a
+// dense thread walks its vertices one after the other, and the last blocks
+// finish late.
+constexpr int kSampleWords = 64;
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
+  __shared__ unsigned short s_list[kSampleWords * 64];
+  __shared__ int s_cnt;
+  const long long words = (p.nslots + 63) / 64;
+  const long long w0 = (long long)blockIdx.x * kSampleWords;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  for (int i = wv; i < kSampleWords; i += 4) {
+    const long long w = w0 + i;
+    if (w >= words) break;
+    const unsigned long long bits = p.actbits[w];
+    if (bits) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&s_cnt, __popcll(bits));
+      base = __shfl(base, 0);
+      if ((bits >> lane) & 1ull) s_list[base + lanes_below(bits)] = (unsigned short)(i * 64 + lane);
     }
-    if (busy) {
-      // one step of meet_link
-      bool done = a == b;
-      if (!done) {
-        if (a < b) {
-          const int t = a;
-          a = b;
-          b = t;
-        }
-        const int na = lab_ld(&ns[a]);
-        if (na != a) {
-          a = na;
-        } else {
-          const int seen = lab_cas(&ns[a], a, b);
-          if (seen == a) done = true;
-          else a = seen;
-        }
-        done = done || a == b;
-      }
-      if (done) {
-        if (b < s0) lab_st(&ns[v], b);
-        busy = false;
-      }
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  for (int j = threadIdx.x; j < cnt; j += 256) {
+    const long long slot = (w0 << 6) + s_list[j];
+    const GraphRef G = resolve<BATCH>(p, slot);
+    const int v = (int)(slot - G.slot0);
+    int* const ns = G.nstat;
+    const int u = p.second[slot];
+    const int s0 = lab_ld(&ns[v]);
+    if (u < v) {
+      const int top = meet_link(s0, lab_ld(&ns[u]), ns);
+      if (top < s0) lab_st(&ns[v], top);
     }
   }
 }
@@ -463,39 +436,39 @@ __global__ __launch_bounds__(256) void k_sample_refill(KParams p) {
 //    with both ends in the tree is hooked by nobody: its ends are already
 //    in one tree.  An edge with one end outside is hooked from that end.
 
+// Writes the high-degree task entries of word w (wave-cooperative).
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
-  const long long ws = wave_slot();
-  if (ws >= p.nslots) return;
-  const long long w = ws >> 6;
-  const unsigned long long hb = p.hubbits[w];
-  const unsigned long long bits = p.lowbits[w];
-  if (!hb && !bits) return;
+__device__ __forceinline__ void emit_hubs(const KParams& p, long long w, unsigned long long hb) {
+  const long long ws = w << 6;
   const GraphRef G = resolve<BATCH>(p, ws);
   const int lane = lane_id();
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
-  if (hb) {
-    const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
-    const bool hub = (hb >> lane) & 1ull;
-    unsigned tasks = 0;
-    int hbeg = 0, hend = 0;
-    if (hub) {
-      hbeg = G.nidx[v];
-      hend = G.nidx[v + 1];
-      tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
-    }
-    const unsigned excl = wave_incl_sum(tasks) - tasks;
-    if (hub) {
-      const long long idx = (long long)(unsigned)base + lanes_below(hb);
-      if (idx < p.hub_cap) {
-        p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl, hbeg, hend};
-      } else {
-        __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+  const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
+  const bool hub = (hb >> lane) & 1ull;
+  unsigned tasks = 0;
+  int hbeg = 0, hend = 0;
+  if (hub) {
+    hbeg = G.nidx[v];
+    hend = G.nidx[v + 1];
+    tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
+  }
+  const unsigned excl = wave_incl_sum(tasks) - tasks;
+  if (hub) {
+    const long long idx = (long long)(unsigned)base + lanes_below(hb);
+    if (idx < p.hub_cap) {
+      p.hubs[idx] = HubEntry{slot, (long long)(unsigned)(base >> 32) + excl, hbeg, hend};
+    } else {
+      __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  if (!((bits >> lane) & 1ull)) return;
+}
+
+// Hooks the list of one low-degree vertex (see above).
+template <bool BATCH>
+__device__ __forceinline__ void hook_low(const KParams& p, long long slot) {
+  const GraphRef G = resolve<BATCH>(p, slot);
+  const int v = (int)(slot - G.slot0);
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
   const int R = p.filter ? p.giant[G.g] : -1;
@@ -526,8 +499,7 @@ __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
     }
     return;
   }
-  const unsigned long long gw = p.filter ? p.gbits[w] : 0ull;
-  const bool marked = (gw >> lane) & 1ull;
+  const bool marked = p.filter && slot_bit(p.gbits, slot);
   int vstat = marked ? R : -1;  // found on first use otherwise
   for (int i = beg; i < end; i += 8) {
     int u[8];
@@ -554,6 +526,43 @@ __global__ __launch_bounds__(256) void k_hook_thread(KParams p) {
       }
     }
   }
+}
+
+// A block covers `span` words (at most kLowWords).  Its wavefronts first
+// write the high-degree task entries of those words, then gather the
+// block's low-degree vertices into an LDS list, and the 256 threads work
+// through the list together: every lane has a vertex, however sparse the
+// work list is (with filter 2 a word holds a few vertices outside the
+// largest tree; R-MAT-24: 0.19 -> 0.12 ms).  The span shrinks for small
+// graphs so that there are still enough blocks to fill the GPU.
+constexpr int kLowWords = 64;
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_hook_thread(KParams p, int span) {
+  __shared__ unsigned short s_list[kLowWords * 64];
+  __shared__ int s_cnt;
+  const long long words = (p.nslots + 63) / 64;
+  const long long w0 = (long long)blockIdx.x * span;
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  for (int i = wv; i < span; i += 4) {
+    const long long w = w0 + i;
+    if (w >= words) break;
+    const unsigned long long hb = p.hubbits[w];
+    if (hb) emit_hubs<BATCH>(p, w, hb);
+    const unsigned long long bits = p.lowbits[w];
+    if (bits) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&s_cnt, __popcll(bits));
+      base = __shfl(base, 0);
+      if ((bits >> lane) & 1ull) s_list[base + lanes_below(bits)] = (unsigned short)(i * 64 + lane);
+    }
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  for (int j = threadIdx.x; j < cnt; j += 256) hook_low<BATCH>(p, (w0 << 6) + s_list[j]);
 }
 
 // ---- high-degree task list: block prefix, grand prefix, emission -----------
@@ -1019,8 +1028,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_init<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
   if constexpr (BATCH) {
-    // batch slot counts are whole chunks: one wavefront per chunk
-    k_sample_refill<<<dim3((unsigned)((p.nslots / kSlotAlign + 3) / 4)), block, 0, s>>>(p);
+    k_sample_dense<true><<<dim3((unsigned)((words + kSampleWords - 1) / kSampleWords)), block, 0, s>>>(p);
   } else {
     k_sample_lane<<<grid, block, 0, s>>>(p);
   }
@@ -1033,7 +1041,10 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_hub_top<<<dim3(1), dim3(1024), 0, s>>>(p);
   mark_ev();
-  k_hook_thread<BATCH><<<grid, block, 0, s>>>(p);
+  // words per compute1 block: as many as keep >= 2048 blocks, 4..kLowWords
+  int span = 4;
+  while (span < kLowWords && words / (2 * span) >= 2048) span *= 2;
+  k_hook_thread<BATCH><<<dim3((unsigned)((words + span - 1) / span)), block, 0, s>>>(p, span);
   mark_ev();
   k_hook_wave<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
