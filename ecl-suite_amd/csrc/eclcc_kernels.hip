@@ -712,11 +712,12 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
 }
 
 // ---- compute2 (ECL-CC-atomic.cu:161-199) --------------------------------------
-// Wavefront w owns bitmap word w, i.e. the 64 slots compute1's wavefront w
-// looked at, and processes all medium-degree vertices among them together:
-// their adjacency lists are laid end to end and dealt out evenly over the
-// 64 lanes, four entries per lane per step, so a wavefront never idles on a
-// short list.  The owner of each entry comes from a per-wavefront LDS table
+// Wavefront w owns the 256-slot chunk w (one graph), gathers its medium-
+// degree vertices into an LDS list, and processes them in groups of up to
+// 64, one per lane: their adjacency lists are laid end to end and dealt out
+// evenly over the 64 lanes, four entries per lane per step, so a wavefront
+// never idles on a short list.  The owner of each entry comes from a
+// per-wavefront LDS table
 // (one byte per entry, filled list by list for a window of kWaveWindow
 // entries); list bases and root estimates of the 64 lanes sit beside it.
 // A vertex already marked as joined to the sampled root uses that root as
@@ -753,111 +754,133 @@ __device__ __forceinline__ void flush_wave(const int2* surv, int cnt, int* ns, i
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   __shared__ unsigned char s_owner[4][kWaveWindow];
+  __shared__ unsigned char s_list[4][kSlotAlign];
   __shared__ int s_base[4][64];
   __shared__ int s_root[4][64];
+  __shared__ int s_vtx[4][64];
   __shared__ int2 s_surv[4][kWaveSurvivors];
-  const long long ws = wave_slot();
-  if (ws >= p.nslots) return;
-  const unsigned long long wbits = p.midbits[ws >> 6];
-  const unsigned long long bits =
-      ((unsigned long long)uniform((int)(wbits >> 32)) << 32) | (unsigned)uniform((int)wbits);
-  if (!bits) return;
-  const GraphRef G = resolve<BATCH>(p, ws);
-  int* const ns = G.nstat;
-  const int* const nl = G.nlist;
+  // one 256-slot chunk (one graph) per wavefront
+  const long long ws0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
+  if (ws0 >= p.nslots) return;
+  const long long nwords = (p.nslots + 63) / 64;
   const int lane = lane_id();
   const int wv = uniform((int)(threadIdx.x >> 6));
+  unsigned char* const list = s_list[wv];
+  int cnt = 0;  // wave-uniform
+#pragma unroll
+  for (int i = 0; i < kSlotAlign / 64; i++) {
+    const long long w = (ws0 >> 6) + i;
+    const unsigned long long wbits = w < nwords ? p.midbits[w] : 0ull;
+    const unsigned long long bits =
+        ((unsigned long long)uniform((int)(wbits >> 32)) << 32) | (unsigned)uniform((int)wbits);
+    if ((bits >> lane) & 1ull) list[cnt + lanes_below(bits)] = (unsigned char)(i * 64 + lane);
+    cnt += __popcll(bits);
+  }
+  if (!cnt) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const GraphRef G = resolve<BATCH>(p, ws0);
+  int* const ns = G.nstat;
+  const int* const nl = G.nlist;
   unsigned char* const owner = s_owner[wv];
   int2* const surv = s_surv[wv];
-  const bool mine = (bits >> lane) & 1ull;
-  const int v = (int)(ws + lane - G.slot0);
   const int R = p.filter ? p.giant[G.g] : -1;
-  const bool skip = p.filter == 2 && R >= 0;  // owners lie outside R's tree
-  const unsigned long long gw = p.filter ? p.gbits[ws >> 6] : 0ull;
-  const int groot = gw ? R : -1;
-  // a marked vertex is joined to the sampled root: that root is a valid
-  // estimate (the CAS of a hook checks it), no walk needed
-  const bool marked = (gw >> lane) & 1ull;
-  int beg = 0, deg = 0, root = 0;
-  if (mine) {
-    root = marked ? groot : lab_ld(&ns[v]);
-    beg = G.nidx[v];
-    deg = G.nidx[v + 1] - beg;
-  }
-  if (mine && !marked) root = find_from(v, root, ns, p.path_mode);
-  const int incl = (int)wave_incl_sum((unsigned)deg);
-  const int off = incl - deg;
-  const int total = __shfl(incl, 63);
-  s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
-  s_root[wv][lane] = root;
-  const int vbase = (int)(ws - G.slot0);
+  const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
   int nsurv = 0;  // wave-uniform
-  for (int w0 = 0; w0 < total; w0 += kWaveWindow) {
-    const int wend = w0 + kWaveWindow < total ? w0 + kWaveWindow : total;
-    for (unsigned long long m = bits; m; m &= m - 1) {
-      const int L = __ffsll((long long)m) - 1;
-      const int oL = __builtin_amdgcn_readlane(off, L);
-      const int dL = __builtin_amdgcn_readlane(deg, L);
-      const int a = oL > w0 ? oL : w0;
-      const int b = oL + dL < wend ? oL + dL : wend;
-      for (int x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
+  // groups of up to 64 listed vertices, one per lane
+  for (int g0 = 0; g0 < cnt; g0 += 64) {
+    const int gn = cnt - g0 < 64 ? cnt - g0 : 64;
+    const unsigned long long bits = gn == 64 ? ~0ull : ((1ull << gn) - 1ull);
+    const bool mine = lane < gn;
+    const long long slot = ws0 + (mine ? list[g0 + lane] : 0);
+    const int v = (int)(slot - G.slot0);
+    // a marked vertex is joined to the sampled root: that root is a valid
+    // estimate (the CAS of a hook checks it), no walk needed
+    const bool marked = mine && p.filter && !skip && slot_bit(p.gbits, slot);
+    const unsigned long long mk = __ballot(marked);
+    int beg = 0, deg = 0, root = 0;
+    if (mine) {
+      root = marked ? R : lab_ld(&ns[v]);
+      beg = G.nidx[v];
+      deg = G.nidx[v + 1] - beg;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int e0 = w0; e0 < wend; e0 += 256) {
-      int u[4], src[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int e = e0 + k * 64 + lane;
-        const bool valid = e < wend;
-        src[k] = valid ? owner[e - w0] : 0;
-        u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
+    if (mine && !marked) root = find_from(v, root, ns, p.path_mode);
+    const int incl = (int)wave_incl_sum((unsigned)deg);
+    const int off = incl - deg;
+    const int total = __shfl(incl, 63);
+    s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
+    s_root[wv][lane] = root;
+    s_vtx[wv][lane] = v;
+    for (int w0 = 0; w0 < total; w0 += kWaveWindow) {
+      const int wend = w0 + kWaveWindow < total ? w0 + kWaveWindow : total;
+      for (unsigned long long m = bits; m; m &= m - 1) {
+        const int L = __ffsll((long long)m) - 1;
+        const int oL = __builtin_amdgcn_readlane(off, L);
+        const int dL = __builtin_amdgcn_readlane(deg, L);
+        const int a = oL > w0 ? oL : w0;
+        const int b = oL + dL < wend ? oL + dL : wend;
+        for (int x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
       }
-      unsigned keep = 0;
-      if (skip) {
-        // every entry; a neighbour inside the largest tree stands for R
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int e0 = w0; e0 < wend; e0 += 256) {
+        int u[4], src[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) keep |= (u[k] != INT_MAX && u[k] != vbase + src[k]) ? (1u << k) : 0u;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) keep |= (vbase + src[k] > u[k]) ? (1u << k) : 0u;
-      }
-      if (gw && !skip) {
-        unsigned in = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (((keep >> k) & 1u) && ((gw >> src[k]) & 1ull)) in |= slot_bit(p.gbits, G.slot0 + u[k]) << k;
-        keep &= ~in;
-      }
-      // stage the entries that still need a hook; flush when the stage is full
-      const unsigned c = __popc(keep);
-      const unsigned pos = wave_incl_sum(c);
-      const int wtot = (int)__shfl(pos, 63);
-      if (wtot) {
-        if (nsurv + wtot > kWaveSurvivors) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          flush_wave(surv, nsurv, ns, p.path_mode);
-          nsurv = 0;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k < 4; k++) {
+          const int e = e0 + k * 64 + lane;
+          const bool valid = e < wend;
+          src[k] = valid ? owner[e - w0] : 0;
+          u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
         }
-        int at = nsurv + (int)(pos - c);
+        unsigned keep = 0;
+        if (skip) {
+          // every entry; a neighbour inside the largest tree stands for R
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
-        nsurv += wtot;
+          for (int k = 0; k < 4; k++)
+            keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
+          if (mk) {
+            unsigned in = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (((keep >> k) & 1u) && ((mk >> src[k]) & 1ull))
+                in |= slot_bit(p.gbits, G.slot0 + u[k]) << k;
+            keep &= ~in;
+          }
+        }
+        // stage the entries that still need a hook; flush when the stage is full
+        const unsigned c = __popc(keep);
+        const unsigned pos = wave_incl_sum(c);
+        const int wtot = (int)__shfl(pos, 63);
+        if (wtot) {
+          if (nsurv + wtot > kWaveSurvivors) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            flush_wave(surv, nsurv, ns, p.path_mode);
+            nsurv = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
+          int at = nsurv + (int)(pos - c);
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
+          nsurv += wtot;
+        }
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
 }
@@ -1046,7 +1069,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   while (span < kLowWords && words / (2 * span) >= 2048) span *= 2;
   k_hook_thread<BATCH><<<dim3((unsigned)((words + span - 1) / span)), block, 0, s>>>(p, span);
   mark_ev();
-  k_hook_wave<BATCH><<<grid, block, 0, s>>>(p);
+  k_hook_wave<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
   mark_ev();
   const dim3 hgrid((unsigned)shape.hub_blocks);
   switch (shape.block_threads) {
