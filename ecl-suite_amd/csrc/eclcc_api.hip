@@ -90,7 +90,7 @@ size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs) {
   const size_t words = (size_t)((nslots + 63) / 64);
   const size_t blocks = (words + kScanBlock - 1) / kScanBlock;
   size_t b = 256;                                                   // counters, flags
-  b += 6 * align_up(sizeof(unsigned long long) * words, 256);  // act, low, mid, hub, info, member words
+  b += 7 * align_up(sizeof(unsigned long long) * words, 256);  // act, smp, low, mid, hub, info, member
   b += align_up(sizeof(unsigned long long) * std::max<size_t>(blocks, 1), 256);
   b += align_up(sizeof(int) * (size_t)std::max<long long>(ngraphs, 1), 256);
   b += align_up(sizeof(int) * (size_t)std::max<long long>(nslots, 1), 256);  // second entries
@@ -108,8 +108,8 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
   w->hubctr = reinterpret_cast<unsigned long long*>(p);
   w->flags = reinterpret_cast<unsigned int*>(p + 128);
   p += 256;
-  unsigned long long** wordarrays[6] = {&w->actbits, &w->lowbits, &w->midbits, &w->hubbits,
-                                        &w->hubinfo,  &w->gbits};
+  unsigned long long** wordarrays[7] = {&w->actbits, &w->smpbits, &w->lowbits, &w->midbits,
+                                        &w->hubbits, &w->hubinfo,  &w->gbits};
   for (auto a : wordarrays) {
     *a = reinterpret_cast<unsigned long long*>(p);
     p += align_up(sizeof(unsigned long long) * words, 256);
@@ -127,6 +127,7 @@ bool carve_workspace(void* base, size_t bytes, long long nslots, long long hub_c
 
 void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
   p->actbits = w.actbits;
+  p->smpbits = w.smpbits;
   p->lowbits = w.lowbits;
   p->midbits = w.midbits;
   p->hubbits = w.hubbits;

@@ -60,6 +60,7 @@ struct KParams {
   const int* chunk_graph;  // batch: graph of each 256-slot chunk
   // one 64-bit word per 64 slots
   unsigned long long* actbits;   // vertices with a smaller neighbour (init)
+  unsigned long long* smpbits;   // vertices with a sampled hook (init -> sample)
   unsigned long long* lowbits;   // low-degree vertices to process (init, final after mark -> compute1)
   unsigned long long* midbits;   // medium-degree vertices to process (-> compute2)
   unsigned long long* hubbits;   // high-degree vertices to process (-> compute1 emits their tasks)
@@ -67,7 +68,7 @@ struct KParams {
   unsigned long long* gbits;     // members of the sampled largest component (mark -> compute2/3)
   unsigned long long* blocktot;  // per kScanBlock words: totals, then exclusive prefix
   int* giant;                    // per graph: sampled root of the largest component, or -1
-  int* second;                   // per slot: second list entry (init -> sample)
+  int* second;                   // per slot: the entry sample hooks (init -> sample)
   HubEntry* hubs;                // high-degree vertices with their first task number
   unsigned long long* hubctr;    // low 32 bits: hubs; high 32: edge tasks
   unsigned int* flags;           // bit 0: hub list overflow
@@ -85,6 +86,7 @@ struct Workspace {
   unsigned long long* hubctr;
   unsigned int* flags;
   unsigned long long* actbits;
+  unsigned long long* smpbits;
   unsigned long long* lowbits;
   unsigned long long* midbits;
   unsigned long long* hubbits;

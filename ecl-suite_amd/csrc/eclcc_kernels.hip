@@ -282,12 +282,12 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
-  int m = v, beg = 0, end = 0, x1 = INT_MAX;
+  int m = v, beg = 0, end = 0, x0 = INT_MAX, x1 = INT_MAX;
   bool longscan = false;
   if (live) {
     beg = G.nidx[v];
     end = G.nidx[v + 1];
-    const int x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
+    x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
     x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
     const int x2 = beg + 2 < end ? G.nlist[beg + 2] : INT_MAX;
     const int x3 = beg + 3 < end ? G.nlist[beg + 3] : INT_MAX;
@@ -313,26 +313,34 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
     }
     if (lane == src) m = found;
   }
+  // the sampled hook: a vertex with a smaller neighbour takes its second
+  // entry if that is smaller too (the first smaller one is its label); a
+  // vertex without one takes its first entry, which nothing else would
+  // join to it before compute1
+  const bool active = live && m != v;
+  const int hooked = active ? x1 : x0;
+  const bool sampled = active ? x1 < v : (live && x0 != INT_MAX && x0 != v);
   if (live) {
     G.nstat[v] = m;  // no other thread reads labels during init
-    p.second[slot] = x1;
+    p.second[slot] = hooked;
   }
   // degree bins: of the vertices that have a smaller neighbour (they hook
   // their entries below them), and with filter 2 of every vertex that has
   // a neighbour (mark then keeps those outside the largest tree)
-  const bool active = live && m != v;
   const bool binned = p.filter == 2 ? live && end > beg : active;
   const int deg = binned ? end - beg : 0;
   const bool low = binned && deg <= p.thread_max;
   const bool mid = deg > p.thread_max && deg <= p.wave_max;
   const bool hub = deg > p.wave_max;
   const unsigned long long actmask = __ballot(active);
+  const unsigned long long smpmask = __ballot(sampled);
   const unsigned long long lowmask = __ballot(low);
   const unsigned long long midmask = __ballot(mid);
   const unsigned long long hubmask = __ballot(hub);
   if (lane == 0) {
     const long long w = ws >> 6;
     p.actbits[w] = actmask;
+    p.smpbits[w] = smpmask;
     p.lowbits[w] = lowmask;
     p.midbits[w] = midmask;
     p.hubbits[w] = hubmask;
@@ -344,34 +352,36 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
 }
 
 // ---- sample: sampled hooks (first part of the reference compute1) -------------
-// One lane per vertex that has a smaller neighbour: hook its second list
-// entry if that is smaller too.  With init's pointer (the first smaller
-// entry) this links random, grid and power-law graphs into one large tree,
-// which mark then finds, so that the remaining edges can be checked against
-// the bitmap.  The sampled edge is hooked again later like every other edge;
-// re-hooking is a no-op.
-// The two paths are walked together (meet_link) and v's label then points at
-// the meeting point: one label store per vertex.  Label stores are
-// write-through on MI355X; a second store (for u) cost more than the shorter
-// walks it saves later (measured on a 1024-graph batch: 13.8 -> 19.3 ms),
-// and walking both paths to their roots (find_from + hook_edge) is slower
-// still (21.6 ms).
+// One hook per vertex, chosen by init: a vertex with a smaller neighbour
+// hooks its second list entry if that is smaller too (the first smaller one
+// is already its label); a vertex without one hooks its first entry, since
+// nothing before compute would join it to anything (R-MAT's degree-1
+// vertices hanging from a larger hub are the common case).  With init's
+// pointers this links random, grid and power-law graphs into one large
+// tree, which mark then finds; with filter 2 the vertices left outside it
+// are all compute has to do.  The sampled edge is hooked again later like
+// every other edge where needed; re-hooking is a no-op.
+// The two paths are walked together (meet_link) and a vertex with a smaller
+// neighbour then points at the meeting point: at most one label store per
+// vertex (a vertex without one is a root; the hook's CAS already moved it).
+// Label stores are write-through on MI355X; a second store (for u) cost
+// more than the shorter walks it saves later (measured on a 1024-graph
+// batch: 13.8 -> 19.3 ms), and walking both paths to their roots
+// (find_from + hook_edge) is slower still (21.6 ms).
 
 // One graph: one lane per vertex.
 __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const unsigned long long act = p.actbits[ws >> 6];
+  const unsigned long long act = p.smpbits[ws >> 6];
   const int lane = lane_id();
   if (!((act >> lane) & 1ull)) return;
   int* const ns = p.nstat;
   const int v = (int)(ws + lane);
   const int u = p.second[v];
   const int s0 = lab_ld(&ns[v]);
-  if (u < v) {
-    const int top = meet_link(s0, lab_ld(&ns[u]), ns);
-    if (top < s0) lab_st(&ns[v], top);
-  }
+  const int top = meet_link(s0, lab_ld(&ns[u]), ns);
+  if (top < s0 && s0 != v) lab_st(&ns[v], top);
 }
 
 // Batches: a block gathers the active vertices of kSampleWords words into
@@ -398,7 +408,7 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
   for (int i = wv; i < kSampleWords; i += 4) {
     const long long w = w0 + i;
     if (w >= words) break;
-    const unsigned long long bits = p.actbits[w];
+    const unsigned long long bits = p.smpbits[w];
     if (bits) {
       int base = 0;
       if (lane == 0) base = atomicAdd(&s_cnt, __popcll(bits));
@@ -415,10 +425,8 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
     int* const ns = G.nstat;
     const int u = p.second[slot];
     const int s0 = lab_ld(&ns[v]);
-    if (u < v) {
-      const int top = meet_link(s0, lab_ld(&ns[u]), ns);
-      if (top < s0) lab_st(&ns[v], top);
-    }
+    const int top = meet_link(s0, lab_ld(&ns[u]), ns);
+    if (top < s0 && s0 != v) lab_st(&ns[v], top);
   }
 }
 
