@@ -231,9 +231,9 @@ const char* eclcc_status_string(int s) {
 const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64)"; }
 
 const char* eclcc_kernel_name(int k) {
-  static const char* names[kNumKernels] = {"init",     "sample",   "pick",     "mark",
-                                           "hubscan",  "hubtop",   "compute1", "compute2",
-                                           "compute3", "flatten"};
+  static const char* names[kNumKernels] = {"init",     "sample",   "pick",
+                                           "mark",     "hubscan",  "compute1",
+                                           "compute2", "compute3", "flatten"};
   return (k >= 0 && k < kNumKernels) ? names[k] : "";
 }
 

@@ -16,8 +16,8 @@
 //   pick      per graph: the most frequent root among 256 sampled vertices
 //   mark      bitmap of the vertices already joined to that root; final work
 //             lists; high-degree counts per word
-//   hubscan   exclusive prefix of the high-degree counts per 1024-word block
-//   hubtop    prefix over the blocks; total hubs and edge tasks
+//   hubscan   exclusive prefix of the high-degree counts per 1024-word block;
+//             the blocks' totals are summed where needed (compute1, compute3)
 //   compute1  writes the high-degree task list; low degree: lane granularity
 //   compute2  medium degree: wavefront granularity
 //   compute3  high degree: block granularity, edge tasks staged through LDS
@@ -262,7 +262,8 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
 
 // ---- init (ECL-CC-atomic.cu:69-88) and degree binning ------------------------
 // label[v] = first neighbour smaller than v in list order, else v.  Each lane
-// checks the first four entries of its list; lists that need a longer scan
+// checks the first two entries of its list, then two more if needed; lists
+// that need a longer scan
 // are finished cooperatively by the whole wavefront, 64 entries per step, the
 // first smaller entry being the lowest set lane of a ballot.
 // The same pass bins the vertices that have a smaller neighbour by degree
@@ -289,9 +290,12 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
     end = G.nidx[v + 1];
     x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
     x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
-    const int x2 = beg + 2 < end ? G.nlist[beg + 2] : INT_MAX;
-    const int x3 = beg + 3 < end ? G.nlist[beg + 3] : INT_MAX;
-    m = x0 < v ? x0 : x1 < v ? x1 : x2 < v ? x2 : x3 < v ? x3 : v;
+    m = x0 < v ? x0 : x1 < v ? x1 : v;
+    if (m == v && end - beg > 2) {  // entries 3 and 4 only when needed
+      const int x2 = G.nlist[beg + 2];
+      const int x3 = beg + 3 < end ? G.nlist[beg + 3] : INT_MAX;
+      m = x2 < v ? x2 : x3 < v ? x3 : v;
+    }
     longscan = (m == v) && (end - beg > 4);
   }
   unsigned long long pending = __ballot(longscan);
@@ -444,15 +448,33 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
 //    with both ends in the tree is hooked by nobody: its ends are already
 //    in one tree.  An edge with one end outside is hooked from that end.
 
-// Writes the high-degree task entries of word w (wave-cooperative).
+// Sum of blocktot[0, nb) over the threads of a block (packed hub and task
+// counts add without carrying into each other).  All threads call it.
+__device__ __forceinline__ unsigned long long sum_block_totals(const KParams& p, long long nb) {
+  __shared__ unsigned long long s_part[16];
+  unsigned long long x = 0;
+  for (long long b = threadIdx.x; b < nb; b += blockDim.x) x += p.blocktot[b];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d);
+  __syncthreads();
+  if (lane_id() == 0) s_part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  unsigned long long t = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += s_part[k];
+  return t;
+}
+
+// Writes the high-degree task entries of word w (wave-cooperative); `gp` is
+// the count of hubs | tasks << 32 in all scan blocks before w's.
 template <bool BATCH>
-__device__ __forceinline__ void emit_hubs(const KParams& p, long long w, unsigned long long hb) {
+__device__ __forceinline__ void emit_hubs(const KParams& p, long long w, unsigned long long hb,
+                                          unsigned long long gp) {
   const long long ws = w << 6;
   const GraphRef G = resolve<BATCH>(p, ws);
   const int lane = lane_id();
   const long long slot = ws + lane;
   const int v = (int)(slot - G.slot0);
-  const unsigned long long base = p.blocktot[w / kScanBlock] + p.hubinfo[w];
+  const unsigned long long base = gp + p.hubinfo[w];
   const bool hub = (hb >> lane) & 1ull;
   unsigned tasks = 0;
   int hbeg = 0, hend = 0;
@@ -548,18 +570,24 @@ constexpr int kLowWords = 64;
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_thread(KParams p, int span) {
   __shared__ unsigned short s_list[kLowWords * 64];
-  __shared__ int s_cnt;
+  __shared__ int s_cnt, s_hubs;
   const long long words = (p.nslots + 63) / 64;
   const long long w0 = (long long)blockIdx.x * span;
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_cnt = 0;
+  if (threadIdx.x == 0) s_cnt = s_hubs = 0;
   __syncthreads();
+  // hub tasks to write? then the counts before this block's scan block
+  // (span divides kScanBlock, so the block lies inside one)
+  for (int i = threadIdx.x; i < span; i += 256)
+    if (w0 + i < words && p.hubbits[w0 + i]) atomicOr(&s_hubs, 1);
+  __syncthreads();
+  const unsigned long long gp = s_hubs ? sum_block_totals(p, w0 / kScanBlock) : 0ull;
   for (int i = wv; i < span; i += 4) {
     const long long w = w0 + i;
     if (w >= words) break;
     const unsigned long long hb = p.hubbits[w];
-    if (hb) emit_hubs<BATCH>(p, w, hb);
+    if (hb) emit_hubs<BATCH>(p, w, hb, gp);
     const unsigned long long bits = p.lowbits[w];
     if (bits) {
       int base = 0;
@@ -613,21 +641,6 @@ __global__ __launch_bounds__(1024) void k_hub_scan(KParams p) {
   const unsigned long long ex = block_excl_scan(x, &tot);
   if (w < words) p.hubinfo[w] = ex;
   if (threadIdx.x == 0) p.blocktot[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(1024) void k_hub_top(KParams p) {
-  const long long words = (p.nslots + 63) / 64;
-  const long long nb = (words + kScanBlock - 1) / kScanBlock;
-  unsigned long long carry = 0;
-  for (long long b0 = 0; b0 < nb; b0 += 1024) {
-    const long long b = b0 + threadIdx.x;
-    const unsigned long long x = b < nb ? p.blocktot[b] : 0ull;
-    unsigned long long tot;
-    const unsigned long long ex = block_excl_scan(x, &tot);
-    if (b < nb) p.blocktot[b] = ex + carry;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *p.hubctr = carry;
 }
 
 // ---- pick: per graph, the most frequent root among sampled vertices -------
@@ -931,7 +944,9 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
   constexpr int CAP = 2 * R * BT;  // staged pairs; room for one more round when below R*BT
   __shared__ int2 stage[CAP];
   __shared__ int s_cnt;
-  const unsigned long long ctr = *p.hubctr;
+  const long long nsb = ((p.nslots + 63) / 64 + kScanBlock - 1) / kScanBlock;
+  const unsigned long long ctr = sum_block_totals(p, nsb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p.hubctr = ctr;  // for eclcc_workspace_stats
   const long long pushed = (long long)(unsigned)ctr;
   const long long nhubs = pushed < p.hub_cap ? pushed : p.hub_cap;
   const long long ntasks = (long long)(ctr >> 32);
@@ -1069,8 +1084,6 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_mark<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
   k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
-  mark_ev();
-  k_hub_top<<<dim3(1), dim3(1024), 0, s>>>(p);
   mark_ev();
   // words per compute1 block: as many as keep >= 2048 blocks, 4..kLowWords
   int span = 4;

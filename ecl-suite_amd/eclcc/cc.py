@@ -19,8 +19,8 @@ import numpy as np
 from . import _native as N
 from .graph import Graph
 
-KERNELS = ("init", "sample", "pick", "mark", "hubscan", "hubtop", "compute1", "compute2",
-           "compute3", "flatten")
+KERNELS = ("init", "sample", "pick", "mark", "hubscan", "compute1", "compute2", "compute3",
+           "flatten")
 
 
 def default_params(**overrides) -> N.EclccParams:

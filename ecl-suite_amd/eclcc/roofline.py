@@ -7,7 +7,7 @@ Per kernel, the bytes its own job cannot avoid moving:
 
     init      4(n+1) + 4n      read the index, write every label
     sample    4 per active vertex (its second list entry)
-    hubscan, hubtop, pick      bookkeeping (no per-vertex or per-edge term)
+    hubscan, pick              bookkeeping (no per-vertex or per-edge term)
     mark      n/8              one bit per vertex
     compute1  4 per adjacency entry + 8 per vertex of the low-degree bin
     compute2  4 per adjacency entry + 8 per vertex of the medium-degree bin
@@ -74,7 +74,6 @@ def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_ma
         "init": 4 * (n + 1) + 4 * n,
         "sample": 4 * b["active"][0],
         "hubscan": 0,
-        "hubtop": 0,
         "pick": 0,
         "mark": (n + 7) // 8,
         "compute1": per_bin["low"],

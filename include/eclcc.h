@@ -105,9 +105,9 @@ int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                      const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
                      const eclcc_params* params, void* stream);
 
-/* Kernels of one run, in launch order: init, sample, hubscan, hubtop, pick,
-   mark, compute1, compute2, compute3, flatten. */
-#define ECLCC_NUM_KERNELS 10
+/* Kernels of one run, in launch order: init, sample, pick, mark, hubscan,
+   compute1, compute2, compute3, flatten. */
+#define ECLCC_NUM_KERNELS 9
 const char* eclcc_kernel_name(int k);
 
 /* As eclcc_run_device, but records HIP events around each kernel, waits,
