@@ -687,8 +687,10 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
 // runs during mark).  With filter 2 the work lists keep only the vertices
 // outside R's tree (each then hooks all of its entries), or, in a graph
 // without a sampled root, the vertices that have a smaller neighbour (each
-// hooks the entries below it, as the reference does).  Also counts each
-// word's high-degree vertices to process and their edge tasks for the scan.
+// hooks the entries below it, as the reference does); the high-degree ones
+// among them are few and go to compute2, which takes lists of any length.
+// Otherwise mark counts each word's high-degree vertices and their edge
+// tasks for the scan.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mark(KParams p) {
@@ -709,7 +711,9 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
   const unsigned long long gw = __ballot(member);
   unsigned long long keep = ~0ull;
   if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
-  const unsigned long long hb = p.hubbits[w] & keep;
+  // filter 2: the few high-degree vertices left join compute2's lists, and
+  // the hub scan and compute3 are not launched
+  const unsigned long long hb = p.filter == 2 ? 0ull : p.hubbits[w] & keep;
   unsigned long long info = 0;
   if (hb) {
     unsigned tasks = 0;
@@ -726,8 +730,8 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
     p.hubinfo[w] = info;
     if (p.filter == 2) {
       p.lowbits[w] &= keep;
-      p.midbits[w] &= keep;
-      p.hubbits[w] = hb;
+      p.midbits[w] = (p.midbits[w] | p.hubbits[w]) & keep;
+      p.hubbits[w] = 0ull;
     }
   }
 }
@@ -1083,7 +1087,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_mark<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
-  k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
+  if (p.filter != 2) k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
   mark_ev();
   // words per compute1 block: as many as keep >= 2048 blocks, 4..kLowWords
   int span = 4;
@@ -1092,11 +1096,13 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_hook_wave<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
   mark_ev();
-  const dim3 hgrid((unsigned)shape.hub_blocks);
-  switch (shape.block_threads) {
-    case 1024: k_hook_block<BATCH, 1024><<<hgrid, dim3(1024), 0, s>>>(p); break;
-    case 512: k_hook_block<BATCH, 512><<<hgrid, dim3(512), 0, s>>>(p); break;
-    default: k_hook_block<BATCH, 256><<<hgrid, dim3(256), 0, s>>>(p); break;
+  if (p.filter != 2) {
+    const dim3 hgrid((unsigned)shape.hub_blocks);
+    switch (shape.block_threads) {
+      case 1024: k_hook_block<BATCH, 1024><<<hgrid, dim3(1024), 0, s>>>(p); break;
+      case 512: k_hook_block<BATCH, 512><<<hgrid, dim3(512), 0, s>>>(p); break;
+      default: k_hook_block<BATCH, 256><<<hgrid, dim3(256), 0, s>>>(p); break;
+    }
   }
   mark_ev();
   k_flatten<BATCH><<<grid, block, 0, s>>>(p);

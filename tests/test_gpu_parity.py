@@ -171,3 +171,25 @@ def test_path_and_filter_modes_on_grid_and_er(cuda, mode, filt):
         announce(f"path_mode={mode} giant_filter={filt} {g.name}")
         labels, _ = compute_timed(g, default_params(path_mode=mode, giant_filter=filt))
         check(g, labels)
+
+
+def _star_plus(n, center, leaves, extra_src=(), extra_dst=()):
+    src = np.concatenate([np.full(len(leaves), center), np.asarray(extra_src, dtype=np.int64)])
+    dst = np.concatenate([np.asarray(leaves), np.asarray(extra_dst, dtype=np.int64)])
+    return G.from_undirected(n, np.stack([src, dst], axis=1))
+
+
+@pytest.mark.parametrize("filt", [1, 2])
+def test_high_degree_vertices_outside_the_largest_tree(cuda, filt):
+    # samples of the root pick sit at multiples of n/256 = 1000; the stars
+    # avoid them, so (a) no graph-wide root is found at all, or (b) a root is
+    # found in a path through the sampled vertices but the star lies outside
+    n = 256000
+    leaves = [v for v in range(1, 3004) if v % 1000]
+    a = _star_plus(n, 255999, leaves)
+    path = [t * 1000 for t in range(256)]
+    b = _star_plus(n, 255999, leaves, path[:-1], path[1:])
+    for g, name in ((a, "no sampled root"), (b, "star outside the sampled tree")):
+        announce(f"giant_filter={filt}: {name}")
+        for kw in (dict(), dict(thread_max_deg=0, wave_max_deg=1), dict(wave_max_deg=100)):
+            check(g, compute(g, default_params(giant_filter=filt, **kw)))
