@@ -692,11 +692,12 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
 // Otherwise mark counts each word's high-degree vertices and their edge
 // tasks for the scan.
 
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_mark(KParams p) {
+// One graph: one lane per vertex (four walks per lane leave too few
+// wavefronts in flight on one graph: R-MAT-24 0.108 vs 0.113 ms).
+__global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const GraphRef G = resolve<BATCH>(p, ws);
+  const GraphRef G = resolve<false>(p, ws);
   const int lane = lane_id();
   const long long w = ws >> 6;
   const long long slot = ws + lane;
@@ -732,6 +733,93 @@ __global__ __launch_bounds__(256) void k_mark(KParams p) {
       p.lowbits[w] &= keep;
       p.midbits[w] = (p.midbits[w] | p.hubbits[w]) & keep;
       p.hubbits[w] = 0ull;
+    }
+  }
+}
+
+// Batches: a wavefront takes one 256-slot chunk (one graph), four vertices
+// per lane, their walks toward R in flight together (the walks are short and
+// bound by load latency, as in flatten; 1024-graph batch: 1.71 -> 1.52 ms).
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_mark(KParams p) {
+  constexpr int K = kSlotAlign / 64;
+  const long long ws0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
+  if (ws0 >= p.nslots) return;
+  const long long nwords = (p.nslots + 63) / 64;
+  const GraphRef G = resolve<BATCH>(p, ws0);
+  int* const ns = G.nstat;
+  const int lane = lane_id();
+  const int R = p.filter ? p.giant[G.g] : -1;
+  int v[K], first[K], cur[K], prev[K];
+  bool live[K], go[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long long slot = ws0 + k * 64 + lane;
+    v[k] = (int)(slot - G.slot0);
+    live[k] = slot < p.nslots && v[k] < G.n;
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) first[k] = (R >= 0 && live[k]) ? lab_ld(&ns[v[k]]) : 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    cur[k] = first[k];
+    prev[k] = v[k];
+    go[k] = R >= 0 && live[k] && first[k] != v[k];
+  }
+  // find_until(v, first, R) for the K vertices at once
+  bool more = true;
+  while (more) {
+    int nx[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      nx[k] = (go[k] && cur[k] != R && cur[k] != 0) ? lab_ld(&ns[cur[k]]) : cur[k];
+    more = false;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (go[k] && cur[k] != R && cur[k] != 0 && cur[k] > nx[k]) {
+        if (p.path_mode == 0) lab_st(&ns[prev[k]], nx[k]);
+        prev[k] = cur[k];
+        cur[k] = nx[k];
+        more = true;
+      } else {
+        go[k] = false;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (p.path_mode != 0 && R >= 0 && live[k] && first[k] != v[k] && cur[k] != first[k])
+      lab_st(&ns[v[k]], cur[k]);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long long w = (ws0 >> 6) + k;
+    if (w >= nwords) break;
+    const bool member = R >= 0 && live[k] && (v[k] == R || (first[k] != v[k] && cur[k] == R));
+    const unsigned long long gw = __ballot(member);
+    unsigned long long keep = ~0ull;
+    if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
+    // filter 2: the few high-degree vertices left join compute2's lists, and
+    // the hub scan and compute3 are not launched
+    const unsigned long long hb = p.filter == 2 ? 0ull : p.hubbits[w] & keep;
+    unsigned long long info = 0;
+    if (hb) {
+      unsigned tasks = 0;
+      if ((hb >> lane) & 1ull) {
+        const int deg = G.nidx[v[k] + 1] - G.nidx[v[k]];
+        tasks = (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk);
+      }
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
+      info = (unsigned long long)__popcll(hb) | ((unsigned long long)tasks << 32);
+    }
+    if (lane == 0) {
+      p.gbits[w] = gw;
+      p.hubinfo[w] = info;
+      if (p.filter == 2) {
+        p.lowbits[w] &= keep;
+        p.midbits[w] = (p.midbits[w] | p.hubbits[w]) & keep;
+        p.hubbits[w] = 0ull;
+      }
     }
   }
 }
@@ -1042,20 +1130,51 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
 
 // ---- flatten (ECL-CC-atomic.cu:250-264) ---------------------------------------
 
+// A wavefront takes one 256-slot chunk (one graph): four vertices per lane,
+// 64 slots apart, their walks in flight together.  The walks are short
+// (mark and compute leave most labels one step from the root), so the pass
+// is bound by load latency and four independent chains per lane hide it
+// (R-MAT-24: 0.033 -> 0.024 ms; grid 256^3: 0.027 -> 0.019 ms).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_flatten(KParams p) {
-  const long long ws = wave_slot();
+  constexpr int K = kSlotAlign / 64;
+  const long long ws = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
   if (ws >= p.nslots) return;
   const GraphRef G = resolve<BATCH>(p, ws);
-  const long long slot = ws + lane_id();
-  const int v = (int)(slot - G.slot0);
-  if (slot < p.nslots && v < G.n) {
-    int* const ns = G.nstat;
-    int s = lab_ld(&ns[v]);
-    const int old = s;
-    int next;
-    while (s != 0 && s > (next = lab_ld(&ns[s]))) s = next;
-    if (s != old) lab_st(&ns[v], s);
+  int* const ns = G.nstat;
+  const int lane = lane_id();
+  int v[K], s[K], old[K];
+  bool go[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long long slot = ws + k * 64 + lane;
+    v[k] = (int)(slot - G.slot0);
+    go[k] = slot < p.nslots && v[k] < G.n;
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) s[k] = go[k] ? lab_ld(&ns[v[k]]) : 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) old[k] = s[k];
+  bool more = true;
+  while (more) {
+    int nx[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) nx[k] = (go[k] && s[k] != 0) ? lab_ld(&ns[s[k]]) : s[k];
+    more = false;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (go[k] && s[k] != 0 && s[k] > nx[k]) {
+        s[k] = nx[k];
+        more = true;
+      } else {
+        go[k] = false;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long long slot = ws + k * 64 + lane;
+    if (slot < p.nslots && v[k] < G.n && s[k] != old[k]) lab_st(&ns[v[k]], s[k]);
   }
 }
 
@@ -1085,7 +1204,11 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
   mark_ev();
-  k_mark<BATCH><<<grid, block, 0, s>>>(p);
+  if constexpr (BATCH) {
+    k_mark<true><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
+  } else {
+    k_mark_lane<<<grid, block, 0, s>>>(p);
+  }
   mark_ev();
   if (p.filter != 2) k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
   mark_ev();
@@ -1105,7 +1228,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
     }
   }
   mark_ev();
-  k_flatten<BATCH><<<grid, block, 0, s>>>(p);
+  k_flatten<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
   mark_ev();
   return hipGetLastError();
 }
