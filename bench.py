@@ -248,7 +248,7 @@ def main() -> None:
     ap.add_argument("--tune-budget", type=float, default=60.0)
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--qps", type=float, default=0.0, help="offered load; 0 = --load x capacity")
-    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--load", type=float, default=0.75)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
     ap.add_argument("--serve-streams", type=int, default=4)

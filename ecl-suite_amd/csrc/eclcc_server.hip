@@ -26,6 +26,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -468,6 +469,59 @@ int64_t eclcc_server_submit_device(eclcc_server* s, int32_t nodes, int32_t edges
                                    const int32_t* d_nindex, const int32_t* d_nlist,
                                    int32_t* d_labels) {
   return submit_request(s, nodes, edges, d_nindex, d_nlist, d_labels, true);
+}
+
+int eclcc_server_submit_device_many(eclcc_server* s, int32_t count, const int32_t* nodes,
+                                    const int32_t* edges, const int32_t* const* d_nindex,
+                                    const int32_t* const* d_nlist, int32_t* const* d_labels,
+                                    int64_t* ids) {
+  if (!s) return set_error(ECLCC_ERR_ARG, "null server");
+  if (count < 0 || (count > 0 && (!nodes || !edges || !d_nindex || !d_nlist || !d_labels || !ids)))
+    return set_error(ECLCC_ERR_ARG, "bad request arguments");
+  // check every request before queueing any
+  for (int32_t i = 0; i < count; i++) {
+    if (nodes[i] < 1 || edges[i] < 0 || !d_nindex[i] || (edges[i] > 0 && !d_nlist[i]) || !d_labels[i])
+      return set_error(ECLCC_ERR_ARG, "bad request arguments (request " + std::to_string(i) + ")");
+    const size_t need = 4096 + request_device_bytes(nodes[i], edges[i], true) +
+                        workspace_bytes(((long long)nodes[i] + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
+                                        hub_capacity(edges[i], s->tuning.wave_max, 1), 1);
+    if (need > s->dev.capacity())
+      return set_error(ECLCC_ERR_BUDGET,
+                       "request " + std::to_string(i) + " larger than the HBM budget");
+  }
+  std::vector<Request*> rs((size_t)count, nullptr);
+  const auto now = Clock::now();
+  for (int32_t i = 0; i < count; i++) {
+    Request* r = new (std::nothrow) Request;
+    if (!r) {
+      for (Request* q : rs) delete q;
+      return set_error(ECLCC_ERR_ALLOC, "host allocation failed");
+    }
+    r->n = nodes[i];
+    r->m = edges[i];
+    r->resident = true;
+    r->nindex = d_nindex[i];
+    r->nlist = d_nlist[i];
+    r->out = d_labels[i];
+    r->t_submit = now;
+    rs[(size_t)i] = r;
+  }
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->stop) {
+      for (Request* q : rs) delete q;
+      return set_error(ECLCC_ERR_STATE, "server is stopping");
+    }
+    for (int32_t i = 0; i < count; i++) {
+      Request* r = rs[(size_t)i];
+      r->id = s->next_id++;
+      ids[i] = r->id;
+      s->reqs[r->id] = r;
+      s->pending.push_back(r);
+    }
+  }
+  s->cv_work.notify_one();
+  return ECLCC_OK;
 }
 
 int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms) {

@@ -141,6 +141,7 @@ def cc_lib() -> C.CDLL:
         L.eclcc_server_submit.restype = i64
         L.eclcc_server_submit_device.argtypes = [vp, i32, i32, vp, vp, vp]
         L.eclcc_server_submit_device.restype = i64
+        L.eclcc_server_submit_device_many.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
         L.eclcc_server_wait.argtypes = [vp, i64, i32, C.POINTER(C.c_double)]
         L.eclcc_server_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64),
                                          C.POINTER(C.c_size_t), C.POINTER(i32)]

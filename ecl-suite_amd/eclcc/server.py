@@ -61,6 +61,32 @@ class Server:
         self._live[int(rid)] = (nindex, nlist, labels)
         return int(rid)
 
+    def submit_device_many(self, reqs) -> list:
+        """Submits device-resident requests together: `reqs` is a sequence of
+        (nodes, edges, nindex, nlist, labels) as for submit_device."""
+        reqs = list(reqs)
+        k = len(reqs)
+        for nodes, edges, ni, nl, lab in reqs:
+            for t, need in ((ni, nodes + 1), (nl, edges), (lab, nodes)):
+                if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch.int32" \
+                        or t.numel() < need:
+                    raise ValueError("nindex/nlist/labels must be contiguous int32 device tensors "
+                                     "of at least nodes+1 / edges / nodes entries")
+        nodes = np.array([r[0] for r in reqs], dtype=np.int32)
+        edges = np.array([r[1] for r in reqs], dtype=np.int32)
+        ptr = lambda i, col: reqs[i][col].data_ptr() if (col != 3 or reqs[i][1]) else 0
+        ni = np.array([ptr(i, 2) for i in range(k)], dtype=np.uint64)
+        nl = np.array([ptr(i, 3) for i in range(k)], dtype=np.uint64)
+        lb = np.array([ptr(i, 4) for i in range(k)], dtype=np.uint64)
+        ids = np.zeros(k, dtype=np.int64)
+        N.check_cc(N.cc_lib().eclcc_server_submit_device_many(
+            self._h, k, nodes.ctypes.data, edges.ctypes.data, ni.ctypes.data, nl.ctypes.data,
+            lb.ctypes.data, ids.ctypes.data))
+        out = [int(x) for x in ids]
+        for rid, (_, _, a, b, c) in zip(out, reqs):
+            self._live[rid] = (a, b, c)
+        return out
+
     def wait(self, rid: int, timeout_ms: int = -1) -> Tuple[np.ndarray, float]:
         lat = C.c_double(0.0)
         rc = N.cc_lib().eclcc_server_wait(self._h, int(rid), int(timeout_ms), C.byref(lat))

@@ -209,6 +209,16 @@ int64_t eclcc_server_submit_device(eclcc_server* s, int32_t nodes, int32_t edges
                                    const int32_t* d_nindex, const int32_t* d_nlist,
                                    int32_t* d_labels);
 
+/* Submits `count` device-resident requests at once (arrays of their sizes
+   and device pointers, as for eclcc_server_submit_device); ids[count]
+   receives their ids.  All are checked before any is queued, and they
+   arrive together, so the worker can pack them into full batches.  Returns
+   0 or a negative status. */
+int eclcc_server_submit_device_many(eclcc_server* s, int32_t count, const int32_t* nodes,
+                                    const int32_t* edges, const int32_t* const* d_nindex,
+                                    const int32_t* const* d_nlist, int32_t* const* d_labels,
+                                    int64_t* ids);
+
 /* Waits for request `id` (timeout_ms < 0: forever).  *latency_ms (optional)
    = submit -> labels in host memory (device memory for a device request). */
 int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms);

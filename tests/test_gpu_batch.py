@@ -130,3 +130,32 @@ def test_server_device_resident_requests(cuda):
     with pytest.raises(ValueError):
         with Server(hbm_budget_bytes=64 << 20, streams=1) as s:
             s.submit_device(gs[0].nodes, gs[0].edges, res[0][0].cpu(), res[0][1], res[0][2])
+
+
+def test_server_bulk_device_requests(cuda):
+    import torch
+
+    gs = small_mix(40, seed=95, n_hi=40000)
+    announce(f"server bulk submission of {len(gs)} device-resident requests")
+    dev = torch.device("cuda:0")
+    store = []
+    for g in gs:
+        ni = torch.from_numpy(np.ascontiguousarray(g.nindex)).to(dev)
+        nl = torch.from_numpy(np.ascontiguousarray(g.nlist)).to(dev) if g.edges else \
+            torch.zeros(1, dtype=torch.int32, device=dev)
+        store.append((g.nodes, g.edges, ni, nl, torch.full((g.nodes,), -1, dtype=torch.int32, device=dev)))
+    torch.cuda.synchronize()
+    with Server(hbm_budget_bytes=128 << 20, streams=2, max_batch=16) as s:
+        for rep in range(2):
+            ids = s.submit_device_many(store)
+            assert len(set(ids)) == len(gs)
+            for g, rid in zip(gs, ids):
+                lab, lat = s.wait(rid, timeout_ms=60000)
+                assert np.array_equal(lab.cpu().numpy(), oracle.cc_labels(g.nindex, g.nlist)), (rep, g.name)
+        assert s.stats()["max_batch"] > 1
+        # a bad request anywhere rejects the whole call before anything is queued
+        done = s.stats()["completed"]
+        bad = list(store[:3]) + [(0, 0, store[0][2], store[0][3], store[0][4])]
+        with pytest.raises(Exception):
+            s.submit_device_many(bad)
+        assert s.stats()["completed"] == done
