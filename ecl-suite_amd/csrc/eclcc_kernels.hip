@@ -12,16 +12,18 @@
 // Kernel sequence (one launch each, no host synchronisation in between):
 //   init      labels := first smaller neighbour (reference init); degree
 //             binning into 64-bit ballot words (no atomics)
-//   sample    hooks of each vertex's second list entry
+//   sample    one sampled hook per vertex
 //   pick      per graph: the most frequent root among 256 sampled vertices
 //   mark      bitmap of the vertices already joined to that root; final work
 //             lists; high-degree counts per word
 //   hubscan   exclusive prefix of the high-degree counts per 1024-word block;
 //             the blocks' totals are summed where needed (compute1, compute3)
-//   compute1  writes the high-degree task list; low degree: lane granularity
+//   compute1  writes the high-degree task list; low degree: thread granularity
 //   compute2  medium degree: wavefront granularity
 //   compute3  high degree: block granularity, edge tasks staged through LDS
 //   flatten   labels := root (reference flatten)
+// With filter 2 hubscan and compute3 are not launched (mark hands the few
+// high-degree vertices left to compute2).
 //
 // Why the bitmap (MI355X measurements): random 4-byte gathers from the
 // 64 MB label array of a 2^24-vertex graph run at ~58 G/s (memory side),
@@ -263,16 +265,16 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
 // ---- init (ECL-CC-atomic.cu:69-88) and degree binning ------------------------
 // label[v] = first neighbour smaller than v in list order, else v.  Each lane
 // checks the first two entries of its list, then two more if needed; lists
-// that need a longer scan
-// are finished cooperatively by the whole wavefront, 64 entries per step, the
-// first smaller entry being the lowest set lane of a ballot.
-// The same pass bins the vertices that have a smaller neighbour by degree
-// (the reference does this in compute1, :116-127): three 64-bit ballot words
-// per wavefront (low, medium, high; no shared worklist: one contended
-// device-scope atomic per push costs ~10 ns, serialised, on MI355X) plus the
-// count and edge-task total of the high-degree ones, and it keeps each
-// vertex's second list entry for the sampled hooks, so that nobody has to
-// fetch the head of every list a second time.
+// that need a longer scan are finished cooperatively by the whole wavefront,
+// 64 entries per step, the first smaller entry being the lowest set lane of
+// a ballot.
+// The same pass bins the vertices by degree (the reference does this in
+// compute1, :116-127) into 64-bit ballot words per wavefront: vertices with
+// a smaller neighbour, vertices with a sampled hook, and low / medium / high
+// degree (no shared worklist: one contended device-scope atomic per push
+// costs ~10 ns, serialised, on MI355X).  It also keeps the entry each vertex
+// hooks in sample, so that nobody has to fetch the head of every list a
+// second time.
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_init(KParams p) {
@@ -388,7 +390,7 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   if (top < s0 && s0 != v) lab_st(&ns[v], top);
 }
 
-// Batches: a block gathers the active vertices of kSampleWords words into
+// Batches: a block gathers the sampled vertices of kSampleWords words into
 // an LDS list and its threads work through it, one meeting walk per thread,
 // so no lane idles on padding slots or inactive vertices and each thread
 // looks up its graph once per vertex.  Measured on a 1024-graph batch of the
@@ -435,9 +437,9 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
 }
 
 // ---- compute1 (rest of ECL-CC-atomic.cu:108-157) -------------------------------
-// Each wavefront first writes the high-degree task entries of its word, in
-// slot order, at the place the scan gave them.  Then low degree
-// (<= thread_max): one lane per vertex, eight list entries in flight per step.
+// Each block first writes the high-degree task entries of its words, in slot
+// order, at the place the scan gave them.  Then low degree (<= thread_max):
+// one thread per vertex, eight list entries in flight per step.
 //  * filter 0/1: the lane hooks the entries smaller than its vertex.  A
 //    vertex marked as joined to the sampled root R takes R as its estimate
 //    and skips every entry whose other end is marked too, after one bitmap
