@@ -211,6 +211,62 @@ def plain(args) -> dict:
     return out
 
 
+def racy(args) -> dict:
+    """The cost of race freedom: the same kernels with plain (racy) label
+    loads and stores (lib/libeclcc_plain.so, a research build) against the
+    product's relaxed atomics, on the same graph and device buffers, for
+    both filter modes.  Labels of the plain build are compared, not trusted."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+    from eclcc import DeviceGraph, default_params
+    from eclcc import _native as N
+
+    g = make_workload(args.workload)
+    log(f"generated {g.name}")
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    P = N.plain_lib()
+    rows = {}
+    for filt in (2, 1):
+        params = default_params(giant_filter=filt)
+        dg = DeviceGraph(g, "cuda:0", params)
+
+        def plain_run():
+            N.check_cc(P.eclcc_run_device(dg.nodes, dg.edges, dg.nindex.data_ptr(),
+                                          dg.nlist.data_ptr(), dg.nstat.data_ptr(),
+                                          dg.work.data_ptr(), dg.work_bytes, C.byref(params),
+                                          stream.cuda_stream))
+
+        def timed(fn):
+            for _ in range(args.warmup):
+                fn()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record(stream)
+            for _ in range(args.steps):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / args.steps
+
+        t_atomic = timed(lambda: dg.run(stream))
+        lab_atomic = dg.labels()
+        t_plain = timed(plain_run)
+        lab_plain = dg.labels()
+        rows[f"giant_filter={filt}"] = {
+            "race_free_ms": t_atomic, "plain_ms": t_plain,
+            "race_free_over_plain": t_atomic / t_plain,
+            "labels_equal": bool(np.array_equal(lab_atomic, lab_plain)),
+        }
+        del dg
+        torch.cuda.empty_cache()
+    return {"metric": METRIC, "mode": "racy", "workload": g.name, "n_gpus": 1,
+            "steps": args.steps, "results": rows,
+            "plain_library": P.eclcc_version().decode()}
+
+
 def copy_probe(args) -> dict:
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -236,7 +292,7 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served", "copy"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served", "copy", "racy"])
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -263,6 +319,8 @@ def main() -> None:
         out = plain(args)
     elif args.mode == "copy":
         out = copy_probe(args)
+    elif args.mode == "racy":
+        out = racy(args)
     else:
         from eclcc import benchmodes
 

@@ -228,7 +228,11 @@ const char* eclcc_status_string(int s) {
   }
 }
 
+#ifndef ECLCC_PLAIN_LABELS
 const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64)"; }
+#else
+const char* eclcc_version(void) { return "eclcc-mi355x 0.2 (gfx950, wave64; plain labels, research)"; }
+#endif
 
 const char* eclcc_kernel_name(int k) {
   static const char* names[kNumKernels] = {"init",     "sample",   "pick",

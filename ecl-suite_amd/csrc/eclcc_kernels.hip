@@ -44,13 +44,23 @@
 namespace eclcc {
 
 // ---- label access: relaxed, agent-scope atomics ---------------------------
+// ECLCC_PLAIN_LABELS builds a research copy (lib/libeclcc_plain.so) whose
+// label loads and stores are plain memory accesses, as in the reference's
+// racy baseline (src/baseline/egr-input/CC/ECL-CC_11.cu:81-93): it measures
+// what race freedom costs on this GPU.  That copy has data races; nothing
+// in the product loads it.  The CAS is atomic in both.
 
+#ifndef ECLCC_PLAIN_LABELS
 __device__ __forceinline__ int lab_ld(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void lab_st(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+#else
+__device__ __forceinline__ int lab_ld(const int* p) { return *p; }
+__device__ __forceinline__ void lab_st(int* p, int v) { *p = v; }
+#endif
 // Returns the value found at *p (== expect iff the swap happened).
 __device__ __forceinline__ int lab_cas(int* p, int expect, int desired) {
   __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,

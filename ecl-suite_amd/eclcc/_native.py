@@ -102,53 +102,69 @@ def graph_lib() -> C.CDLL:
     return _graph
 
 
+def _declare_cc(L: C.CDLL) -> C.CDLL:
+    PP = C.POINTER(EclccParams)
+    L.eclcc_last_error.restype = C.c_char_p
+    L.eclcc_status_string.argtypes = [C.c_int]
+    L.eclcc_status_string.restype = C.c_char_p
+    L.eclcc_version.restype = C.c_char_p
+    L.eclcc_kernel_name.argtypes = [C.c_int]
+    L.eclcc_kernel_name.restype = C.c_char_p
+    L.eclcc_params_default.argtypes = [PP]
+    L.eclcc_params_default.restype = None
+    L.eclcc_compute.argtypes = [i32, i32, vp, vp, vp]
+    L.eclcc_compute_timed.argtypes = [i32, i32, vp, vp, vp, PP, C.POINTER(C.c_double)]
+    L.eclcc_workspace_bytes.argtypes = [i32, i64, PP]
+    L.eclcc_workspace_bytes.restype = C.c_size_t
+    L.eclcc_run_device.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp]
+    L.eclcc_run_device_profiled.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp,
+                                            C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.eclcc_run_init_only.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, vp]
+    L.eclcc_workspace_stats.argtypes = [vp, C.c_size_t, vp, C.POINTER(i64), C.POINTER(i64)]
+    L.eclcc_batch_create.argtypes = [i32, vp, vp, vp, vp, PP, C.POINTER(vp)]
+    L.eclcc_batch_run.argtypes = [vp, vp, C.c_int]
+    L.eclcc_batch_run_single.argtypes = [vp, i32, vp]
+    L.eclcc_batch_set_params.argtypes = [vp, PP]
+    L.eclcc_batch_run_profiled.argtypes = [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.eclcc_batch_labels.argtypes = [vp, i32, vp]
+    L.eclcc_batch_device_labels.argtypes = [vp, i32]
+    L.eclcc_batch_device_labels.restype = vp
+    L.eclcc_batch_device_bytes.argtypes = [vp]
+    L.eclcc_batch_device_bytes.restype = C.c_size_t
+    L.eclcc_batch_free.argtypes = [vp]
+    L.eclcc_batch_free.restype = None
+    L.eclcc_server_create.argtypes = [C.POINTER(EclccServerConfig), PP, C.POINTER(vp)]
+    L.eclcc_server_submit.argtypes = [vp, i32, i32, vp, vp, vp]
+    L.eclcc_server_submit.restype = i64
+    L.eclcc_server_submit_device.argtypes = [vp, i32, i32, vp, vp, vp]
+    L.eclcc_server_submit_device.restype = i64
+    L.eclcc_server_submit_device_many.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+    L.eclcc_server_wait.argtypes = [vp, i64, i32, C.POINTER(C.c_double)]
+    L.eclcc_server_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64),
+                                     C.POINTER(C.c_size_t), C.POINTER(i32)]
+    L.eclcc_server_free.argtypes = [vp]
+    L.eclcc_server_free.restype = None
+    return L
+
+
 def cc_lib() -> C.CDLL:
     global _cc
     if _cc is None:
-        L = _load("libeclcc.so")
-        PP = C.POINTER(EclccParams)
-        L.eclcc_last_error.restype = C.c_char_p
-        L.eclcc_status_string.argtypes = [C.c_int]
-        L.eclcc_status_string.restype = C.c_char_p
-        L.eclcc_version.restype = C.c_char_p
-        L.eclcc_kernel_name.argtypes = [C.c_int]
-        L.eclcc_kernel_name.restype = C.c_char_p
-        L.eclcc_params_default.argtypes = [PP]
-        L.eclcc_params_default.restype = None
-        L.eclcc_compute.argtypes = [i32, i32, vp, vp, vp]
-        L.eclcc_compute_timed.argtypes = [i32, i32, vp, vp, vp, PP, C.POINTER(C.c_double)]
-        L.eclcc_workspace_bytes.argtypes = [i32, i64, PP]
-        L.eclcc_workspace_bytes.restype = C.c_size_t
-        L.eclcc_run_device.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp]
-        L.eclcc_run_device_profiled.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp,
-                                                C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.eclcc_run_init_only.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, vp]
-        L.eclcc_workspace_stats.argtypes = [vp, C.c_size_t, vp, C.POINTER(i64), C.POINTER(i64)]
-        L.eclcc_batch_create.argtypes = [i32, vp, vp, vp, vp, PP, C.POINTER(vp)]
-        L.eclcc_batch_run.argtypes = [vp, vp, C.c_int]
-        L.eclcc_batch_run_single.argtypes = [vp, i32, vp]
-        L.eclcc_batch_set_params.argtypes = [vp, PP]
-        L.eclcc_batch_run_profiled.argtypes = [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.eclcc_batch_labels.argtypes = [vp, i32, vp]
-        L.eclcc_batch_device_labels.argtypes = [vp, i32]
-        L.eclcc_batch_device_labels.restype = vp
-        L.eclcc_batch_device_bytes.argtypes = [vp]
-        L.eclcc_batch_device_bytes.restype = C.c_size_t
-        L.eclcc_batch_free.argtypes = [vp]
-        L.eclcc_batch_free.restype = None
-        L.eclcc_server_create.argtypes = [C.POINTER(EclccServerConfig), PP, C.POINTER(vp)]
-        L.eclcc_server_submit.argtypes = [vp, i32, i32, vp, vp, vp]
-        L.eclcc_server_submit.restype = i64
-        L.eclcc_server_submit_device.argtypes = [vp, i32, i32, vp, vp, vp]
-        L.eclcc_server_submit_device.restype = i64
-        L.eclcc_server_submit_device_many.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
-        L.eclcc_server_wait.argtypes = [vp, i64, i32, C.POINTER(C.c_double)]
-        L.eclcc_server_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64),
-                                         C.POINTER(C.c_size_t), C.POINTER(i32)]
-        L.eclcc_server_free.argtypes = [vp]
-        L.eclcc_server_free.restype = None
-        _cc = L
+        _cc = _declare_cc(_load("libeclcc.so"))
     return _cc
+
+
+_plain = None
+
+
+def plain_lib() -> C.CDLL:
+    """Research copy of the library whose label loads and stores are plain
+    (racy) memory accesses, built only to measure what race freedom costs.
+    Nothing else in the package uses it."""
+    global _plain
+    if _plain is None:
+        _plain = _declare_cc(_load("libeclcc_plain.so"))
+    return _plain
 
 
 class EclccError(RuntimeError):

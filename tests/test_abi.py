@@ -84,6 +84,12 @@ def test_label_accesses_go_through_atomic_helpers():
             continue
         line = body[body.rfind("\n", 0, m.start()) + 1:body.find("\n", m.end())]
         plain.append(line.strip())
+    # the product's helpers are relaxed agent-scope atomics (the plain ones
+    # exist only under ECLCC_PLAIN_LABELS, for the research copy)
+    prod = body.split("#ifndef ECLCC_PLAIN_LABELS", 1)[1].split("#else", 1)[0]
+    assert "__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)" in prod
+    assert "__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)" in prod
+    assert "plain labels" not in N.cc_lib().eclcc_version().decode()
     # the one plain store is init's, where no other thread reads labels
     assert plain == ["G.nstat[v] = m;  // no other thread reads labels during init"] or \
         plain == ["G.nstat[v] = m;"], plain
