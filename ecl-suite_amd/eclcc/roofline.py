@@ -6,7 +6,7 @@ once):  B(n, m) = 4(n+1) + 4m + 4n.
 Per kernel, the bytes its own job cannot avoid moving:
 
     init      4(n+1) + 4n      read the index, write every label
-    sample    4 per active vertex (its second list entry)
+    sample    4 per sampled vertex (the entry it hooks)
     hubscan, pick              bookkeeping (no per-vertex or per-edge term)
     mark      n/8              one bit per vertex
     compute1  4 per adjacency entry + 8 per vertex of the low-degree bin
@@ -15,7 +15,10 @@ Per kernel, the bytes its own job cannot avoid moving:
     flatten   8n               read and write every label
 
 A vertex is active when it has a smaller neighbour (only those hook,
-ECL-CC-atomic.cu:119-132); the bins split the active vertices by degree:
+ECL-CC-atomic.cu:119-132).  The sample kernel hooks one entry per vertex:
+the second entry of an active vertex if that is smaller, the first entry
+of an inactive vertex with a neighbour.  The bins split the active
+vertices by degree:
 low <= thread_max_deg < medium <= wave_max_deg < high.  That is the work of
 giant_filter 0 and 1.  With giant_filter 2 the bins hold only the vertices
 outside the sampled largest tree, which the device decides during the run:
@@ -29,7 +32,7 @@ import numpy as np
 
 PER_UNIT = {
     "init": "4(n+1) + 4n bytes per graph",
-    "sample": "4 bytes per active vertex",
+    "sample": "4 bytes per sampled vertex",
     "mark": "1/8 byte per vertex",
     "compute1": "4 bytes per entry + 8 bytes per vertex of the low-degree bin",
     "compute2": "4 bytes per entry + 8 bytes per vertex of the medium-degree bin",
@@ -51,12 +54,20 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) 
     if nlist.shape[0]:
         starts = nindex[:-1][has]
         smallest[has] = np.minimum.reduceat(nlist, starts)
-    active = smallest < np.arange(n)
+    ids = np.arange(n)
+    active = smallest < ids
+    first = np.full(n, np.iinfo(np.int32).max, dtype=np.int64)
+    second = np.full(n, np.iinfo(np.int32).max, dtype=np.int64)
+    first[has] = nlist[nindex[:-1][has]]
+    two = deg >= 2
+    second[two] = nlist[nindex[:-1][two] + 1]
+    sampled = np.where(active, second < ids, has & (first != ids))
     low = active & (deg <= thread_max)
     mid = active & (deg > thread_max) & (deg <= wave_max)
     hub = active & (deg > wave_max)
     return {
         "active": (int(active.sum()), int(deg[active].sum())),
+        "sampled": (int(sampled.sum()), int(deg[sampled].sum())),
         "low": (int(low.sum()), int(deg[low].sum())),
         "mid": (int(mid.sum()), int(deg[mid].sum())),
         "hub": (int(hub.sum()), int(deg[hub].sum())),
@@ -72,7 +83,7 @@ def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_ma
         per_bin.update(low=None, mid=None, hub=None)
     return {
         "init": 4 * (n + 1) + 4 * n,
-        "sample": 4 * b["active"][0],
+        "sample": 4 * b["sampled"][0],
         "hubscan": 0,
         "pick": 0,
         "mark": (n + 7) // 8,

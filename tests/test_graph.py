@@ -230,3 +230,20 @@ def test_kernel_bytes_partition_the_active_entries():
     s = Graph.from_edges(3000, np.arange(2999), np.full(2999, 2999))
     b = roofline.bins(s.nindex, s.nlist, 16, 2048)
     assert b["hub"] == (1, 2999) and b["low"] == (0, 0)
+
+
+def test_sampled_vertex_count_matches_the_rule():
+    from eclcc import roofline
+
+    g = workloads.rmat(11, 8, seed=5)
+    want = 0
+    for v in range(g.nodes):
+        lst = g.nlist[g.nindex[v]:g.nindex[v + 1]]
+        if len(lst) == 0:
+            continue
+        if lst.min() < v:  # active: its second entry, if smaller
+            want += int(len(lst) >= 2 and lst[1] < v)
+        else:              # inactive: its first entry
+            want += int(lst[0] != v)
+    assert roofline.bins(g.nindex, g.nlist, 16, 2048)["sampled"][0] == want
+    assert roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048)["sample"] == 4 * want
