@@ -94,6 +94,37 @@ __device__ __forceinline__ unsigned wave_incl_sum(unsigned x) {
   return x;
 }
 
+// Loads the work-list words w0 .. w0 + span - 1 (span <= 64) of a block
+// at once, by its first wavefront, into s_bits, and places each word's
+// vertices in the block's LDS list by a prefix sum of their counts (s_off;
+// s_cnt = the total).  One global load latency for the whole block instead
+// of one per word.  The caller synchronises the block afterwards.
+__device__ __forceinline__ unsigned long long gather_words(const unsigned long long* bits,
+                                                           long long w0, int span, long long words,
+                                                           unsigned long long* s_bits, int* s_off,
+                                                           int* s_cnt) {
+  const int i = threadIdx.x;
+  const long long w = w0 + i;
+  const unsigned long long b = (i < span && w < words) ? bits[w] : 0ull;
+  s_bits[i] = b;
+  const unsigned c = (unsigned)__popcll(b);
+  const unsigned incl = wave_incl_sum(c);
+  s_off[i] = (int)(incl - c);
+  if (i == 63) *s_cnt = (int)incl;
+  return b;
+}
+
+// Writes the vertices of the gathered words into the LDS list, one word per
+// wavefront at a time (after gather_words and a block barrier).
+__device__ __forceinline__ void list_words(const unsigned long long* s_bits, const int* s_off,
+                                           int span, unsigned short* s_list) {
+  const int lane = lane_id();
+  for (int i = (int)(threadIdx.x >> 6); i < span; i += (int)(blockDim.x >> 6)) {
+    const unsigned long long b = s_bits[i];
+    if ((b >> lane) & 1ull) s_list[s_off[i] + lanes_below(b)] = (unsigned short)(i * 64 + lane);
+  }
+}
+
 // bit `slot` of a per-slot bitmap
 __device__ __forceinline__ unsigned slot_bit(const unsigned long long* bits, long long slot) {
   const unsigned* w = reinterpret_cast<const unsigned*>(bits);
@@ -413,25 +444,16 @@ constexpr int kSampleWords = 64;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_sample_dense(KParams p) {
+  static_assert(kSampleWords == 64, "one word per lane of the first wavefront");
   __shared__ unsigned short s_list[kSampleWords * 64];
+  __shared__ unsigned long long s_bits[kSampleWords];
+  __shared__ int s_off[kSampleWords];
   __shared__ int s_cnt;
   const long long words = (p.nslots + 63) / 64;
   const long long w0 = (long long)blockIdx.x * kSampleWords;
-  const int lane = lane_id();
-  const int wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_cnt = 0;
+  if (threadIdx.x < 64) (void)gather_words(p.smpbits, w0, kSampleWords, words, s_bits, s_off, &s_cnt);
   __syncthreads();
-  for (int i = wv; i < kSampleWords; i += 4) {
-    const long long w = w0 + i;
-    if (w >= words) break;
-    const unsigned long long bits = p.smpbits[w];
-    if (bits) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&s_cnt, __popcll(bits));
-      base = __shfl(base, 0);
-      if ((bits >> lane) & 1ull) s_list[base + lanes_below(bits)] = (unsigned short)(i * 64 + lane);
-    }
-  }
+  list_words(s_bits, s_off, kSampleWords, s_list);
   __syncthreads();
   const int cnt = s_cnt;
   for (int j = threadIdx.x; j < cnt; j += 256) {
@@ -517,7 +539,13 @@ __device__ __forceinline__ void hook_low(const KParams& p, long long slot) {
   const int beg = G.nidx[v];
   const int end = G.nidx[v + 1];
   if (p.filter == 2 && R >= 0) {
+    // Every neighbour inside R's tree stands for R, so the edges into the
+    // tree all join the same two trees: the first one is hooked, the rest
+    // are no-ops and are dropped (they would each cost a label load and a
+    // step of the lane's sequential hook chain).
+    const int s = lab_ld(&ns[v]);
     int vstat = -1;  // found on first use
+    bool joined = false;
     for (int i = beg; i < end; i += 8) {
       int u[8];
 #pragma unroll
@@ -526,15 +554,20 @@ __device__ __forceinline__ void hook_low(const KParams& p, long long slot) {
 #pragma unroll
       for (int k = 0; k < 8; k++) need[k] = u[k] >= 0 && u[k] != v;
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (need[k] && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
+      for (int k = 0; k < 8; k++) {
+        if (need[k] && slot_bit(p.gbits, G.slot0 + u[k])) {
+          u[k] = R;
+          need[k] = !joined;
+          joined = true;
+        }
+      }
       int sv[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) sv[k] = need[k] ? lab_ld(&ns[u[k]]) : 0;
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         if (need[k]) {
-          if (vstat < 0) vstat = find_root(v, ns, p.path_mode);
+          if (vstat < 0) vstat = find_from(v, s, ns, p.path_mode);
           vstat = hook_edge(vstat, u[k], sv[k], ns, p.path_mode);
         }
       }
@@ -581,33 +614,31 @@ constexpr int kLowWords = 64;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_hook_thread(KParams p, int span) {
+  static_assert(kLowWords == 64, "one word per lane of the first wavefront");
   __shared__ unsigned short s_list[kLowWords * 64];
+  __shared__ unsigned long long s_bits[kLowWords], s_hub[kLowWords];
+  __shared__ int s_off[kLowWords];
   __shared__ int s_cnt, s_hubs;
   const long long words = (p.nslots + 63) / 64;
   const long long w0 = (long long)blockIdx.x * span;
-  const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_cnt = s_hubs = 0;
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+    const unsigned long long hb = (i < span && w0 + i < words) ? p.hubbits[w0 + i] : 0ull;
+    (void)gather_words(p.lowbits, w0, span, words, s_bits, s_off, &s_cnt);
+    s_hub[i] = hb;
+    const unsigned long long any = __ballot(hb != 0ull);
+    if (i == 0) s_hubs = any != 0ull;
+  }
   __syncthreads();
   // hub tasks to write? then the counts before this block's scan block
   // (span divides kScanBlock, so the block lies inside one)
-  for (int i = threadIdx.x; i < span; i += 256)
-    if (w0 + i < words && p.hubbits[w0 + i]) atomicOr(&s_hubs, 1);
-  __syncthreads();
-  const unsigned long long gp = s_hubs ? sum_block_totals(p, w0 / kScanBlock) : 0ull;
-  for (int i = wv; i < span; i += 4) {
-    const long long w = w0 + i;
-    if (w >= words) break;
-    const unsigned long long hb = p.hubbits[w];
-    if (hb) emit_hubs<BATCH>(p, w, hb, gp);
-    const unsigned long long bits = p.lowbits[w];
-    if (bits) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&s_cnt, __popcll(bits));
-      base = __shfl(base, 0);
-      if ((bits >> lane) & 1ull) s_list[base + lanes_below(bits)] = (unsigned short)(i * 64 + lane);
-    }
+  if (s_hubs) {
+    const unsigned long long gp = sum_block_totals(p, w0 / kScanBlock);
+    for (int i = wv; i < span && w0 + i < words; i += 4)
+      if (s_hub[i]) emit_hubs<BATCH>(p, w0 + i, s_hub[i], gp);
   }
+  list_words(s_bits, s_off, span, s_list);
   __syncthreads();
   const int cnt = s_cnt;
   for (int j = threadIdx.x; j < cnt; j += 256) hook_low<BATCH>(p, (w0 << 6) + s_list[j]);
@@ -913,6 +944,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   const int R = p.filter ? p.giant[G.g] : -1;
   const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
   int nsurv = 0;  // wave-uniform
+  unsigned long long intree = 0;  // per lane: owners seen with an entry inside R's tree
   // groups of up to 64 listed vertices, one per lane
   for (int g0 = 0; g0 < cnt; g0 += 64) {
     const int gn = cnt - g0 < 64 ? cnt - g0 : 64;
@@ -961,13 +993,19 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
         }
         unsigned keep = 0;
         if (skip) {
-          // every entry; a neighbour inside the largest tree stands for R
+          // every entry; a neighbour inside the largest tree stands for R,
+          // and all such entries of one vertex join the same two trees: the
+          // owner is noted and hooked to R once, after its list
 #pragma unroll
           for (int k = 0; k < 4; k++)
             keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
 #pragma unroll
-          for (int k = 0; k < 4; k++)
-            if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) u[k] = R;
+          for (int k = 0; k < 4; k++) {
+            if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) {
+              keep &= ~(1u << k);
+              intree |= 1ull << src[k];
+            }
+          }
         } else {
 #pragma unroll
           for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
@@ -1005,6 +1043,13 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (skip) {
+      // one hook to R per listed vertex with a neighbour in R's tree
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) intree |= __shfl_xor(intree, d);
+      if ((intree >> lane) & 1ull) (void)hook_edge(root, R, lab_ld(&ns[R]), ns, p.path_mode);
+      intree = 0;
     }
   }
   if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
