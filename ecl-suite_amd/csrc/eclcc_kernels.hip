@@ -305,10 +305,13 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
 
 // ---- init (ECL-CC-atomic.cu:69-88) and degree binning ------------------------
 // label[v] = first neighbour smaller than v in list order, else v.  Each lane
-// checks the first two entries of its list, then two more if needed; lists
-// that need a longer scan are finished cooperatively by the whole wavefront,
-// 64 entries per step, the first smaller entry being the lowest set lane of
-// a ballot.
+// checks the first two entries of its list; the lanes that need more scan
+// on together, eight entries of their own list per step (in R-MAT-24 the
+// vertices with no smaller neighbour, which must read their whole list,
+// crowd into the low IDs: some wavefronts hold dozens of them); lists
+// still unfinished after kLaneScan entries are finished cooperatively by
+// the whole wavefront, 64 entries per step, the first smaller entry being
+// the lowest set lane of a ballot.
 // The same pass bins the vertices by degree (the reference does this in
 // compute1, :116-127) into 64-bit ballot words per wavefront: vertices with
 // a smaller neighbour, vertices with a sampled hook, and low / medium / high
@@ -316,6 +319,8 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
 // costs ~10 ns, serialised, on MI355X).  It also keeps the entry each vertex
 // hooks in sample, so that nobody has to fetch the head of every list a
 // second time.
+
+constexpr int kLaneScan = 2 + 3 * 8;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_init(KParams p) {
@@ -327,26 +332,33 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
   int m = v, beg = 0, end = 0, x0 = INT_MAX, x1 = INT_MAX;
-  bool longscan = false;
   if (live) {
     beg = G.nidx[v];
     end = G.nidx[v + 1];
     x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
     x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
     m = x0 < v ? x0 : x1 < v ? x1 : v;
-    if (m == v && end - beg > 2) {  // entries 3 and 4 only when needed
-      const int x2 = G.nlist[beg + 2];
-      const int x3 = beg + 3 < end ? G.nlist[beg + 3] : INT_MAX;
-      m = x2 < v ? x2 : x3 < v ? x3 : v;
-    }
-    longscan = (m == v) && (end - beg > 4);
   }
+  int pos = beg + 2;
+  bool scan = live && m == v && pos < end;
+  while (__ballot(scan)) {
+    if (scan) {
+      int x[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) x[k] = pos + k < end ? G.nlist[pos + k] : INT_MAX;
+#pragma unroll
+      for (int k = 0; k < 8; k++) m = (m == v && x[k] < v) ? x[k] : m;
+      pos += 8;
+      scan = m == v && pos < end && pos < beg + kLaneScan;
+    }
+  }
+  const bool longscan = live && m == v && pos < end;
   unsigned long long pending = __ballot(longscan);
   while (pending) {
     const int src = __ffsll((long long)pending) - 1;
     pending &= pending - 1;
     const int sv = __shfl(v, src);
-    const int sb = __shfl(beg, src) + 4;
+    const int sb = __shfl(pos, src);
     const int se = __shfl(end, src);
     int found = sv;
     for (int base = sb; base < se; base += 64) {

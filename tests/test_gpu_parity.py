@@ -100,6 +100,37 @@ def test_init_kernel_closed_form(cuda):
     check(g, dg.labels())
 
 
+def test_init_finds_the_smaller_neighbour_at_every_list_position(cuda):
+    # vertex 1000 + i has 40 entries, all larger than it except vertex i,
+    # placed at position i: covers the first two entries, the per-lane
+    # steps and the cooperative scan of longer lists
+    from eclcc.graph import Graph
+    L = 40
+    src, dst = [], []
+    for i in range(L):
+        hub = 1000 + i
+        src.append(hub); dst.append(i)
+        for j in range(L - 1):
+            src.append(hub); dst.append(2000 + 64 * i + j)
+    g = Graph.from_edges(2000 + 64 * L, np.array(src), np.array(dst))
+    nl = g.nlist.copy()
+    for i in range(L):
+        hub = 1000 + i
+        b, e = g.nindex[hub], g.nindex[hub + 1]
+        lst = [x for x in nl[b:e] if x != i]
+        lst.insert(i, i)
+        nl[b:e] = lst
+    g.nlist = nl
+    want = oracle.init_labels(g.nindex, g.nlist)
+    assert all(want[1000 + i] == i for i in range(L))
+    announce("init only, smaller neighbour at positions 0..39")
+    dg = DeviceGraph(g, "cuda:0")
+    dg.run_init_only()
+    assert np.array_equal(dg.labels(), want)
+    dg.run()
+    check(g, dg.labels())
+
+
 def test_device_runs_are_repeatable(cuda):
     g = workloads.rmat(16, 16, 12)
     announce(f"repeat {g.name}")
