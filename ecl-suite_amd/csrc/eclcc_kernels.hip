@@ -432,13 +432,17 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
 __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   const long long ws = wave_slot();
   if (ws >= p.nslots) return;
-  const unsigned long long act = p.smpbits[ws >> 6];
   const int lane = lane_id();
-  if (!((act >> lane) & 1ull)) return;
   int* const ns = p.nstat;
   const int v = (int)(ws + lane);
-  const int u = p.second[v];
-  const int s0 = lab_ld(&ns[v]);
+  // the lane's entry and label are loaded together with the word that says
+  // whether it hooks at all (one latency less; the lines are read anyway
+  // whenever a lane of the wavefront hooks)
+  const int vi = v < p.n ? v : 0;  // lanes past the end load vertex 0's
+  const int u = p.second[vi];
+  const int s0 = lab_ld(&ns[vi]);
+  const unsigned long long act = p.smpbits[ws >> 6];
+  if (!((act >> lane) & 1ull)) return;
   const int top = meet_link(s0, lab_ld(&ns[u]), ns);
   if (top < s0 && s0 != v) lab_st(&ns[v], top);
 }
@@ -759,17 +763,19 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   const int v = (int)(slot - G.slot0);
   const bool live = slot < p.nslots && v < G.n;
   const int R = p.filter ? p.giant[G.g] : -1;
+  // the word's lists and the lane's label, loaded before the walks so that
+  // their latency hides behind them
+  const unsigned long long act = p.actbits[w], low = p.lowbits[w], mid = p.midbits[w],
+                           hub = p.hubbits[w];
+  const int s = lab_ld(&G.nstat[live ? v : 0]);
   bool member = false;
-  if (R >= 0 && live) {
-    const int s = lab_ld(&G.nstat[v]);
-    member = v == R || find_until(v, s, R, G.nstat, p.path_mode) == R;
-  }
+  if (R >= 0 && live) member = v == R || find_until(v, s, R, G.nstat, p.path_mode) == R;
   const unsigned long long gw = __ballot(member);
   unsigned long long keep = ~0ull;
-  if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
+  if (p.filter == 2) keep = R >= 0 ? ~gw : act;
   // filter 2: the few high-degree vertices left join compute2's lists, and
   // the hub scan and compute3 are not launched
-  const unsigned long long hb = p.filter == 2 ? 0ull : p.hubbits[w] & keep;
+  const unsigned long long hb = p.filter == 2 ? 0ull : hub & keep;
   unsigned long long info = 0;
   if (hb) {
     unsigned tasks = 0;
@@ -785,8 +791,8 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
     p.gbits[w] = gw;
     p.hubinfo[w] = info;
     if (p.filter == 2) {
-      p.lowbits[w] &= keep;
-      p.midbits[w] = (p.midbits[w] | p.hubbits[w]) & keep;
+      p.lowbits[w] = low & keep;
+      p.midbits[w] = (mid | hub) & keep;
       p.hubbits[w] = 0ull;
     }
   }
