@@ -468,11 +468,10 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
 // 256-slot chunk), 5.6 ms dense.  On one large graph the per-lane kernel is
 // faster (R-MAT-24: 0.20 vs 0.23 ms; 3-D grid 256^3: 0.67 vs 0.73 ms): a
 // dense thread walks its vertices one after the other, and the last blocks
-// finish late.
-#ifndef ECLCC_SAMPLE_WORDS
-#define ECLCC_SAMPLE_WORDS 64
-#endif
-constexpr int kSampleWords = ECLCC_SAMPLE_WORDS;
+// finish late.  Blocks of fewer words are slower on batches (1024 graphs,
+// sample with 8 / 16 / 32 words per block: +90 / +55 / +20 %): the blocks
+// that run together then cover fewer graphs (see spread_block).
+constexpr int kSampleWords = 64;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride) {
