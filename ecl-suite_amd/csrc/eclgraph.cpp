@@ -312,6 +312,24 @@ int eclgraph_write(const eclgraph_csr* g, const char* path) {
   return ECLGRAPH_OK;
 }
 
+int eclgraph_mst_weights(int32_t nodes, const int32_t* nindex, const int32_t* nlist,
+                         int32_t* eweight) {
+  if (nodes < 1 || !nindex || (nindex[nodes] > 0 && (!nlist || !eweight)))
+    return fail(ECLGRAPH_ERR_ARG, "bad argument");
+  parallel_for(nodes, 1 << 14, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++) {
+      for (int32_t j = nindex[i]; j < nindex[i + 1]; j++) {
+        // int product of the reference, formed without signed overflow
+        const int32_t prod = (int32_t)((uint32_t)i * (uint32_t)nlist[j]);
+        int32_t w = 1 + prod % nodes;
+        if (w < 0) w = -w;
+        eweight[j] = w;
+      }
+    }
+  });
+  return ECLGRAPH_OK;
+}
+
 int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const int32_t* nlist,
                    int flags, int64_t* where) {
   if (where) *where = -1;

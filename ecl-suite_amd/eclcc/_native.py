@@ -1,9 +1,10 @@
-"""ctypes bindings of the two in-tree libraries.
+"""ctypes bindings of the in-tree libraries.
 
-libeclcc.so    -- HIP kernels + C ABI (include/eclcc.h)
+libeclcc.so    -- CC: HIP kernels + C ABI (include/eclcc.h)
+libeclmst.so   -- MST: HIP kernels + C ABI (include/eclmst.h)
 libeclgraph.so -- host graph I/O and generators (include/eclgraph.h)
 
-Both are built in-tree by ``make -C ecl-suite_amd`` (``__graft_entry__.build()``).
+All are built in-tree by ``make -C ecl-suite_amd`` (``__graft_entry__.build()``).
 There is no fallback: if a library is missing, loading raises.
 """
 from __future__ import annotations
@@ -98,6 +99,7 @@ def graph_lib() -> C.CDLL:
         L.eclgraph_rand64.restype = u64
         L.eclgraph_set_threads.argtypes = [C.c_int]
         L.eclgraph_set_threads.restype = None
+        L.eclgraph_mst_weights.argtypes = [i32, vp, vp, vp]
         _graph = L
     return _graph
 
@@ -167,6 +169,39 @@ def plain_lib() -> C.CDLL:
     return _plain
 
 
+_mst = None
+
+
+def mst_lib() -> C.CDLL:
+    global _mst
+    if _mst is None:
+        L = _load("libeclmst.so")
+        L.eclmst_last_error.restype = C.c_char_p
+        L.eclmst_version.restype = C.c_char_p
+        L.eclmst_threshold.argtypes = [i32, i32, vp]
+        L.eclmst_threshold.restype = i32
+        L.eclmst_compute.argtypes = [i32, i32, vp, vp, vp, vp]
+        L.eclmst_compute_timed.argtypes = [i32, i32, vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.eclmst_workspace_bytes.argtypes = [i32, i32]
+        L.eclmst_workspace_bytes.restype = C.c_size_t
+        L.eclmst_run_device.argtypes = [i32, i32, vp, vp, vp, vp, vp, C.c_size_t, i32, vp,
+                                        C.POINTER(C.c_float)]
+        _mst = L
+    return _mst
+
+
+class EclmstError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"eclmst status {status}: {message}")
+        self.status = status
+
+
+def check_mst(rc: int) -> int:
+    if rc < 0:
+        raise EclmstError(rc, (mst_lib().eclmst_last_error() or b"").decode())
+    return rc
+
+
 class EclccError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"eclcc status {status}: {message}")
@@ -200,7 +235,7 @@ def header_functions(header: str) -> list[str]:
     text = (INCLUDE_DIR / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
-    names = re.findall(r"\b((?:eclcc|eclgraph)_[a-z0-9_]+)\s*\(", text)
+    names = re.findall(r"\b((?:eclcc|eclmst|eclgraph)_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
 
 

@@ -47,6 +47,14 @@ enum {
 int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const int32_t* nlist,
                    int flags, int64_t* where);
 
+/* Default edge weights of the reference's ECL-MST main
+   (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu:421-430): the entry j of
+   vertex i with neighbour k weighs 1 + ((i * k) % nodes), negated if
+   negative, with i * k wrapping in 32-bit two's complement as the
+   reference's int product does.  eweight gets nindex[nodes] entries. */
+int eclgraph_mst_weights(int32_t nodes, const int32_t* nindex, const int32_t* nlist,
+                         int32_t* eweight);
+
 /* Host CSR owned by the library; arrays are read through the accessors. */
 typedef struct eclgraph_csr eclgraph_csr;
 

@@ -90,6 +90,29 @@ int oracle_eclcc_threads(int32_t n, const int32_t* nindex, const int32_t* nlist,
 /* Number of distinct values among label[0..n) that lie in [0, n). */
 int64_t oracle_count_components(int32_t n, const int32_t* label);
 
+/* ---- ECL-MST (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu) ----------
+   What the output must be: in_mst[j] = 1 for the entry j (vertex i, neighbour
+   k = nlist[j] > i) of every edge of the minimum spanning forest under the
+   strict order (weight, j); 0 for every other entry, including the k < i
+   direction.  The reference's GPU code keys each edge by weight << 32 | j
+   (:249-251) and its CPU Kruskal sorts by the tuple (weight, j, i, k)
+   (:121-131), so both pick this one forest.
+   Pinned (tests/test_mst.py): the forest's total weight equals
+   scipy.sparse.csgraph.minimum_spanning_tree's on random graphs; the edge
+   set equals the brute-force minimum over all spanning forests of the keys
+   weight * 2^32 + j on every graph of 5 vertices with small weights; it has
+   n - (number of components) edges and joins each component; closed forms
+   for a tree (every edge) and for a cycle (all but the largest key). */
+
+/* Default weights (ECL-MST_10-atomic.cu:421-430). */
+void oracle_mst_weights(int32_t n, const int32_t* nindex, const int32_t* nlist, int32_t* w);
+
+/* Kruskal (ECL-MST_10-atomic.cu:113-154).  Weights must be >= 0.  Returns 0,
+   -1 on bad arguments, -2 on an ID out of range or a negative weight, -3 if
+   out of memory. */
+int oracle_mst(int32_t n, const int32_t* nindex, const int32_t* nlist, const int32_t* w,
+               uint8_t* in_mst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,4 +1,4 @@
-"""ctypes face of liboracle.so -- the CPU oracle of the ECL-CC path.
+"""ctypes face of liboracle.so -- the CPU oracle of the ECL-CC and ECL-MST paths.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, and only as the checker (or as the CPU baseline
@@ -50,6 +50,10 @@ def lib() -> C.CDLL:
         L.oracle_eclcc_threads.restype = C.c_int
         L.oracle_count_components.argtypes = [i32, vp]
         L.oracle_count_components.restype = C.c_int64
+        L.oracle_mst_weights.argtypes = [i32, vp, vp, vp]
+        L.oracle_mst_weights.restype = None
+        L.oracle_mst.argtypes = [i32, vp, vp, vp, vp]
+        L.oracle_mst.restype = C.c_int
         _lib = L
     return _lib
 
@@ -112,3 +116,25 @@ def verify(nindex, nlist, labels):
 def count_components(labels) -> int:
     lab = np.ascontiguousarray(labels, dtype=np.int32)
     return int(lib().oracle_count_components(lab.shape[0], lab.ctypes.data))
+
+
+def mst_weights(nindex, nlist) -> np.ndarray:
+    """The reference's default edge weights (ECL-MST_10-atomic.cu:421-430)."""
+    ni, nl, n = _csr(nindex, nlist)
+    w = np.empty(nl.shape[0], dtype=np.int32)
+    lib().oracle_mst_weights(n, ni.ctypes.data, nl.ctypes.data if nl.size else None,
+                             w.ctypes.data if w.size else None)
+    return w
+
+
+def mst(nindex, nlist, weights) -> np.ndarray:
+    """in_mst[j] (uint8) of the minimum spanning forest under (weight, j), by
+    Kruskal (ECL-MST_10-atomic.cu:113-154)."""
+    ni, nl, n = _csr(nindex, nlist)
+    w = np.ascontiguousarray(weights, dtype=np.int32)
+    out = np.empty(max(nl.shape[0], 1), dtype=np.uint8)
+    rc = lib().oracle_mst(n, ni.ctypes.data, nl.ctypes.data if nl.size else None,
+                          w.ctypes.data if w.size else None, out.ctypes.data)
+    if rc < 0:
+        raise RuntimeError(f"oracle_mst failed with {rc}")
+    return out[: nl.shape[0]]

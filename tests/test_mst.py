@@ -1,0 +1,221 @@
+"""ECL-MST minimum spanning forest: the oracle pinned on the CPU, the
+library's exports and host logic, and (marked gpu) the HIP path against the
+oracle.  Reference: src/racefree/egr-input/MST/ECL-MST_10-atomic.cu."""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import announce
+from eclcc import Graph, workloads
+from eclcc import _native as N
+from eclcc import mst
+from oracle import oracle
+
+INT32_MAX = 2**31 - 1
+
+
+def from_pairs(n, pairs):
+    a = np.array([p[0] for p in pairs], dtype=np.int32)
+    b = np.array([p[1] for p in pairs], dtype=np.int32)
+    return Graph.from_edges(n, a, b)
+
+
+def key_weights(g, w):
+    """{entry j: key} of the one-direction entries (nlist[j] > i)."""
+    out = {}
+    for i in range(g.nodes):
+        for j in range(g.nindex[i], g.nindex[i + 1]):
+            if g.nlist[j] > i:
+                out[j] = (int(w[j]) << 32) | j
+    return out
+
+
+# ---- the oracle, pinned -----------------------------------------------------
+
+def test_default_weights_hand_values():
+    # 1 + ((i * k) % n), negated if negative, i * k in 32-bit two's complement
+    g = from_pairs(70000, [(2, 3), (46341, 65536), (65536, 69999)])
+    w = oracle.mst_weights(g.nindex, g.nlist)
+    want = {(2, 3): 1 + 6, (3, 2): 1 + 6,
+            # 46341 * 65536 = 3037003776 wraps to -1257963520; % 70000 = -63520
+            (46341, 65536): 63519, (65536, 46341): 63519,
+            # 65536 * 69999 = 4587454464 wraps to 292487168; % 70000 = 27168
+            (65536, 69999): 27169, (69999, 65536): 27169}
+    for i in range(g.nodes):
+        for j in range(g.nindex[i], g.nindex[i + 1]):
+            assert w[j] == want[(i, int(g.nlist[j]))]
+    # the library computes the same weights
+    assert np.array_equal(mst.mst_weights(g), w)
+
+
+def test_oracle_weight_equals_scipy():
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import minimum_spanning_tree, connected_components
+
+    rng = np.random.default_rng(3)
+    for t in range(12):
+        g = workloads.rmat(9, 4, t) if t % 2 else Graph.er(400, 1200, t)
+        # scipy drops zero weights: keep them >= 1; many ties
+        w = np.empty(g.edges, dtype=np.int32)
+        for i in range(g.nodes):
+            for j in range(g.nindex[i], g.nindex[i + 1]):
+                k = g.nlist[j]
+                w[j] = 1 + (min(i, k) * 7 + max(i, k) * 13 + t) % 5
+        t_in = oracle.mst(g.nindex, g.nlist, w)
+        A = sp.csr_matrix((w.astype(np.float64), g.nlist, g.nindex), shape=(g.nodes, g.nodes))
+        assert int(minimum_spanning_tree(A).sum()) == int(w[t_in == 1].astype(np.int64).sum())
+        ncomp = connected_components(A, directed=False)[0]
+        assert int(t_in.sum()) == g.nodes - ncomp
+
+
+def brute_force_forest(g, w):
+    """Entries of the spanning forest with the least sum of keys, by trying
+    every subset of the edges of a tiny graph."""
+    keys = key_weights(g, w)
+    ends = {}
+    for i in range(g.nodes):
+        for j in range(g.nindex[i], g.nindex[i + 1]):
+            if g.nlist[j] > i:
+                ends[j] = (i, int(g.nlist[j]))
+    # components of the whole graph
+    lab = oracle.cc_labels(g.nindex, g.nlist)
+    need = g.nodes - len(set(lab.tolist()))
+    best, best_sum = None, None
+    for sub in itertools.combinations(sorted(ends), need):
+        parent = list(range(g.nodes))
+
+        def f(x):
+            while parent[x] != x:
+                x = parent[x]
+            return x
+        ok = True
+        for j in sub:
+            a, b = f(ends[j][0]), f(ends[j][1])
+            if a == b:
+                ok = False
+                break
+            parent[max(a, b)] = min(a, b)
+        if ok:
+            s = sum(keys[j] for j in sub)
+            if best_sum is None or s < best_sum:
+                best, best_sum = set(sub), s
+    return best
+
+
+def test_oracle_equals_brute_force_on_five_vertices():
+    all_pairs = list(itertools.combinations(range(5), 2))
+    rng = np.random.default_rng(11)
+    for mask in range(1, 1 << len(all_pairs), 3):
+        pairs = [p for b, p in enumerate(all_pairs) if mask >> b & 1]
+        g = from_pairs(5, pairs)
+        w = rng.integers(0, 3, g.edges).astype(np.int32)  # ties everywhere
+        t_in = oracle.mst(g.nindex, g.nlist, w)
+        assert set(np.nonzero(t_in)[0].tolist()) == brute_force_forest(g, w), pairs
+
+
+def test_oracle_closed_forms():
+    # a tree: every edge, from its smaller end only
+    g = workloads.rmat(8, 1, 1)
+    lab = oracle.cc_labels(g.nindex, g.nlist)
+    tree_pairs = [(v, int(lab[v])) for v in range(g.nodes) if lab[v] != v]  # star per component
+    t = from_pairs(g.nodes, tree_pairs)
+    w = np.arange(t.edges, dtype=np.int32) % 7
+    t_in = oracle.mst(t.nindex, t.nlist, w)
+    for i in range(t.nodes):
+        for j in range(t.nindex[i], t.nindex[i + 1]):
+            assert t_in[j] == (t.nlist[j] > i)
+    # a cycle: every edge but the one with the largest key
+    n = 9
+    c = from_pairs(n, [(v, (v + 1) % n) for v in range(n)])
+    w = np.full(c.edges, 4, dtype=np.int32)
+    t_in = oracle.mst(c.nindex, c.nlist, w)
+    keys = key_weights(c, w)
+    assert set(np.nonzero(t_in)[0].tolist()) == set(keys) - {max(keys, key=keys.get)}
+    # no edges
+    e = Graph(nindex=np.zeros(4, dtype=np.int32), nlist=np.zeros(0, dtype=np.int32))
+    assert oracle.mst(e.nindex, e.nlist, np.zeros(0, dtype=np.int32)).size == 0
+
+
+def test_threshold_host_logic():
+    g = workloads.rmat(12, 8, 2)
+    w = mst.mst_weights(g)
+    thr = mst.threshold(g, w)
+    s = np.sort(w[(np.arange(256, dtype=np.int64) * g.edges) // 256])
+    assert thr == s[min(255, int(3.0 * g.nodes * 256 / g.edges))]
+    sparse = workloads.grid3d(8, 0.6, 1)  # average degree < 4: one phase
+    assert mst.threshold(sparse, mst.mst_weights(sparse)) == INT32_MAX
+
+
+def test_mst_argument_errors_are_statuses():
+    L = N.mst_lib()
+    assert L.eclmst_compute(0, 0, None, None, None, None) == -1
+    assert L.eclmst_workspace_bytes(0, 0) == 0
+    assert L.eclmst_run_device(5, 0, None, None, None, None, None, 0, -1, None, None) == -1
+
+
+# ---- the HIP path against the oracle ------------------------------------------
+
+def _cases():
+    yield "er", Graph.er(3000, 12000, 5), None
+    yield "rmat", workloads.rmat(12, 8, 7), None
+    yield "grid", workloads.grid3d(16, 0.6, 2), None
+    yield "rmat-ties", workloads.rmat(11, 16, 9), "ties"
+    yield "isolated", workloads.rmat(12, 1, 3), None
+    yield "er-dense", Graph.er(1000, 40000, 6), "ties"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,g,wk", list(_cases()), ids=[c[0] for c in _cases()])
+def test_gpu_mst_matches_oracle(cuda, name, g, wk):
+    w = mst.mst_weights(g)
+    if wk == "ties":
+        w = (w % 4).astype(np.int32)
+    announce(f"mst {name} n={g.nodes} m={g.edges}")
+    want = oracle.mst(g.nindex, g.nlist, w)
+    got = mst.compute_mst(g, w)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_gpu_mst_device_thresholds_agree(cuda):
+    g = workloads.rmat(14, 8, 4)
+    w = mst.mst_weights(g)
+    want = oracle.mst(g.nindex, g.nlist, w)
+    dm = mst.DeviceMST(g, w)
+    for thr in (-1, INT32_MAX, 0, int(np.median(w)), mst.threshold(g, w)):
+        announce(f"mst device threshold {thr}")
+        dm.run(thr)
+        assert np.array_equal(dm.result(), want), thr
+
+
+@pytest.mark.gpu
+def test_gpu_mst_medium_rmat(cuda):
+    g = workloads.rmat(18, 16, 5)
+    w = mst.mst_weights(g)
+    announce(f"mst rmat18 n={g.nodes} m={g.edges}")
+    want = oracle.mst(g.nindex, g.nlist, w)
+    got, sec = mst.compute_mst(g, w, timed=True)
+    assert np.array_equal(got, want)
+    assert sec > 0
+
+
+@pytest.mark.gpu
+def test_gpu_mst_rejects_bad_inputs(cuda):
+    g = workloads.rmat(10, 8, 1)
+    w = mst.mst_weights(g)
+    w[5] = -3
+    with pytest.raises(N.EclmstError) as e:
+        mst.compute_mst(g, w)
+    assert e.value.status == -6
+    # one direction only (not symmetric): more list entries than the lists hold
+    n = 2000
+    nindex = np.arange(0, 3 * n + 1, 3, dtype=np.int32)
+    nlist = np.array([min(i + d, n - 1) if i + d < n else i for i in range(n) for d in (1, 2, 3)],
+                     dtype=np.int32)
+    gd = Graph(nindex=nindex, nlist=nlist)
+    with pytest.raises(N.EclmstError) as e:
+        mst.compute_mst(gd, np.ones(nlist.shape[0], dtype=np.int32))
+    assert e.value.status == -6
