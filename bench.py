@@ -10,8 +10,10 @@ pass, and the CPU baseline (the oracle's threaded ECL-CC restatement on the
 host cores, same graph).
 
 Other modes: ``--mode batched`` (many graphs, one hipGraph launch, against
-one graph per launch) and ``--mode served`` (continuous batching under
-Poisson arrivals, p50/p99 latency).
+one graph per launch), ``--mode served`` (continuous batching under
+Poisson arrivals, p50/p99 latency), ``--mode racy`` (the cost of race
+freedom) and ``--mode mst`` (the ECL-MST minimum spanning forest of the
+same graph with the reference's default weights).
 """
 from __future__ import annotations
 
@@ -267,6 +269,105 @@ def racy(args) -> dict:
             "plain_library": P.eclcc_version().decode()}
 
 
+def mst_bench(args) -> dict:
+    """ECL-MST (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu) on the same
+    workload with main's default weights (:421-430) and threshold (:446-459).
+    One step = one forest, graph resident in HBM; the time is the device
+    window from the first kernel to the last, which includes the per-round
+    reads of the worklist size, as the reference's GPUTimer does (:339-375).
+    Checked at full size: the forest has n - c edges and the same components
+    as the graph (c from the CC library); checked edge for edge against the
+    oracle's Kruskal on R-MAT-21, which is also the CPU baseline."""
+    import numpy as np
+    import torch
+    from eclcc import DeviceGraph, Graph, mst, workloads
+
+    t0 = time.perf_counter()
+    g = make_workload(args.workload)
+    w = mst.mst_weights(g)
+    thr = mst.threshold(g, w)
+    log(f"generated {g.name} with weights in {time.perf_counter() - t0:.1f}s; threshold {thr}")
+    torch.cuda.set_device(0)
+    dm = mst.DeviceMST(g, w)
+    for _ in range(args.warmup):
+        dm.run(thr)
+    torch.cuda.synchronize()
+    log("timed region (mst)")
+    dev_ms, wall_ms = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        dev_ms.append(dm.run(thr))
+        wall_ms.append((time.perf_counter() - t1) * 1e3)
+    ms = statistics.mean(dev_ms)
+    forest = dm.result()
+    del dm
+    torch.cuda.empty_cache()
+    # spanning-forest check at full size, through the CC library
+    dg = DeviceGraph(g, "cuda:0")
+    dg.run()
+    torch.cuda.synchronize()
+    lab = dg.labels()
+    del dg
+    ncomp = int(np.unique(lab).size)
+    src = np.repeat(np.arange(g.nodes, dtype=np.int32), np.diff(g.nindex))
+    sel = forest == 1
+    fg = Graph.from_edges(g.nodes, src[sel], g.nlist[sel])
+    df = DeviceGraph(fg, "cuda:0")
+    df.run()
+    torch.cuda.synchronize()
+    spanning = bool(int(sel.sum()) == g.nodes - ncomp and np.array_equal(df.labels(), lab))
+    del df
+    weight = int(w[sel].astype(np.int64).sum())
+    # edge-for-edge parity and the CPU baseline on a bounded sample
+    gs = workloads.rmat(21)
+    ws = mst.mst_weights(gs)
+    from oracle import oracle
+
+    times = []
+    ref = None
+    t_end = time.perf_counter() + args.cpu_budget
+    while not times or (len(times) < 3 and time.perf_counter() < t_end):
+        t1 = time.perf_counter()
+        ref = oracle.mst(gs.nindex, gs.nlist, ws)
+        times.append(time.perf_counter() - t1)
+    got_s = mst.compute_mst(gs, ws)
+    B = 4 * (g.nodes + 1) + 8 * g.edges + g.edges
+    return {
+        "metric": "MST edges/s on RMAT-24 (next path after CC)",
+        "mode": "mst",
+        "value": g.edges / (ms * 1e-3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "wall_ms_per_step": statistics.mean(wall_ms),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{g.name}, default weights (ECL-MST_10-atomic.cu:421-430)",
+                   "nodes": g.nodes, "edges": g.edges, "threshold": thr,
+                   "algorithmic_bytes": B},
+        "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
+        "forest_edges": int(sel.sum()),
+        "forest_weight": weight,
+        "components": ncomp,
+        "spanning_forest": spanning,
+        "sample_parity": {"workload": gs.name, "edges_equal_oracle": bool(np.array_equal(got_s, ref))},
+        "cpu_baseline": {
+            "value": gs.edges / statistics.median(times),
+            "unit": "edges/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{gs.name} (n={gs.nodes}, m={gs.edges}), the oracle's Kruskal "
+                      f"(ECL-MST_10-atomic.cu:113-154), median of {len(times)} runs",
+            "seconds": statistics.median(times),
+        },
+    }
+
+
 def copy_probe(args) -> dict:
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -292,7 +393,8 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "served", "copy", "racy"])
+    ap.add_argument("--mode", default="plain",
+                    choices=["plain", "batched", "served", "copy", "racy", "mst"])
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -321,6 +423,8 @@ def main() -> None:
         out = copy_probe(args)
     elif args.mode == "racy":
         out = racy(args)
+    elif args.mode == "mst":
+        out = mst_bench(args)
     else:
         from eclcc import benchmodes
 

@@ -20,14 +20,17 @@
 // What is MI355X-specific:
 //   * No per-edge or per-wave atomic on a shared list counter: a contended
 //     device-scope atomic executes at the memory side and serialises
-//     (~10 ns each).  Blocks are persistent and append through a 2048-entry
-//     LDS buffer, one counter atomic per buffer flush.
+//     (~10 ns each).  Blocks are persistent and append through a 1024-entry
+//     LDS buffer, one counter atomic per buffer flush; 17 KB of LDS per
+//     block leaves room for 8 blocks (32 wavefronts) per CU.
 //   * The first lists are built edge-parallel over the CSR (no thread owns a
 //     hub's whole list): each block takes a contiguous range of entries,
 //     finds the vertex of its first entry by a 64-way parallel search of
 //     nindex, then maps entries to vertices through an LDS window of nindex.
 //   * Between the light and the heavy phase every vertex is pointed at its
-//     root, so the heavy phase's finds take one or two loads.
+//     root, so the heavy phase reads each end's root with one load.
+//   * A component minimum is reset only if it is not reset already: the
+//     stores of a large tree's edges would all hit its root's entry.
 //   * Race freedom by construction: parent and minv are only touched through
 //     relaxed agent-scope atomics; each in_mst byte has one writer.
 
@@ -47,8 +50,8 @@ namespace eclmst {
 constexpr int kThreads = 256;
 constexpr int kPer = 4;                      // items per thread per step
 constexpr int kTile = kThreads * kPer;       // items per block step
-constexpr int kBuf = 2048;                   // LDS output buffer (int4 entries)
-constexpr int kWin = 1024;                   // vertices in the LDS nindex window
+constexpr int kBuf = kTile;                  // LDS output buffer (int4 entries)
+constexpr int kWin = 256;                    // vertices in the LDS nindex window
 constexpr int kSamples = 256;                // weights sampled for the threshold
 constexpr unsigned long long kNoEdge = ~0ull;
 
@@ -60,6 +63,7 @@ struct Ctrl {
   unsigned count[2];  // worklist sizes
   unsigned flags;
   int threshold;
+  int giant;  // root of the largest sampled tree after the light phase, or -1
 };
 
 // ---- atomics (relaxed, agent scope) ----------------------------------------
@@ -108,6 +112,11 @@ __device__ __forceinline__ void join(int a, int b, int* parent) {
     if (seen == a) return;
     a = seen;
   }
+}
+
+// bit v of a membership bitmap (written by an earlier kernel)
+__device__ __forceinline__ bool member(const unsigned long long* gbits, int v) {
+  return (gbits[v >> 6] >> (v & 63)) & 1ull;
 }
 
 __device__ __forceinline__ unsigned long long key_of(int4 e) {
@@ -232,18 +241,20 @@ __device__ __forceinline__ int window_slot(const int* s, long long key) {
 }
 
 // initWL (:188-237), edge-parallel.  HEAVY = false: the entries i -> k with
-// k > i and weight <= threshold, as (i, k, w, j).  HEAVY = true: those with
-// weight > threshold whose ends lie in different trees, as (root i, root k,
-// w, j).  Every weight is checked for sign.
+// k > i and weight <= threshold, as (i, k, w, j).  HEAVY = true (after
+// k_flatten): those with weight > threshold whose ends lie in different
+// trees, as (root i, root k, w, j).  Every weight is checked for sign.
 template <bool HEAVY>
 __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const int* nindex,
                                                     const int* nlist, const int* eweight,
-                                                    int* parent, Ctrl* ctrl, int4* wl,
-                                                    unsigned cap, long long per) {
+                                                    int* parent, const unsigned long long* gbits,
+                                                    Ctrl* ctrl, int4* wl, unsigned cap,
+                                                    long long per) {
   __shared__ Appender A;
   __shared__ int s_idx[kWin + 1];
   __shared__ int s_i0;
   const int thr = ctrl->threshold;
+  const bool giant = HEAVY && ctrl->giant >= 0;
   const long long jb0 = (long long)blockIdx.x * per;
   const long long jb1 = std::min<long long>(m, jb0 + per);
   if (threadIdx.x == 0) A.fill = 0;
@@ -291,9 +302,13 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
           neg |= w < 0;
           if (k > i && (HEAVY ? w > thr : w <= thr)) {
             if (HEAVY) {
-              const int a = find(i, parent), b = find(k, parent);
-              ok[r] = a != b;
-              it[r] = make_int4(a, b, w, (int)j);
+              // after k_flatten every vertex points at its root, and no
+              // union runs during this kernel
+              if (!(giant && member(gbits, i) && member(gbits, k))) {
+                const int a = ld(&parent[i]), b = ld(&parent[k]);
+                ok[r] = a != b;
+                it[r] = make_int4(a, b, w, (int)j);
+              }
             } else {
               ok[r] = true;
               it[r] = make_int4(i, k, w, (int)j);
@@ -370,15 +385,59 @@ __global__ __launch_bounds__(kThreads) void k_join(const int4* wl, const unsigne
   }
 }
 
-// kernel3 (:274-282): the listed edges' trees get fresh minima.
-__global__ __launch_bounds__(kThreads) void k_reset(const int4* wl, const unsigned* cnt,
-                                                    unsigned cap, unsigned long long* minv) {
-  const long long total = std::min<long long>(*cnt, cap);
+// kernel3 (:274-282) resets the minima of the listed edges' trees, two
+// scattered stores per edge; the edges of a large tree all name its root,
+// and agent-scope stores to one address serialise at the memory side
+// (R-MAT-24: 10.4 ms per round; 0.3-0.7 ms with a load before each store).
+// Here the whole array is reset with coalesced 16-byte stores (8n bytes,
+// ~25 us for 2^24 vertices).  No other kernel runs meanwhile.
+__global__ __launch_bounds__(kThreads) void k_reset(int n, unsigned long long* minv) {
   const long long stride = (long long)gridDim.x * kThreads;
-  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < total; i += stride) {
-    const int4 e = wl[i];
-    st64(&minv[e.x], kNoEdge);
-    st64(&minv[e.y], kNoEdge);
+  uint4* w = reinterpret_cast<uint4*>(minv);  // minv is 256-byte aligned
+  const uint4 none = make_uint4(~0u, ~0u, ~0u, ~0u);
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n / 2; i += stride) w[i] = none;
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) minv[n - 1] = kNoEdge;
+}
+
+// Root of the largest tree after the light phase (parents flattened): the
+// most frequent root among kSamples evenly spaced vertices, if it occurs
+// at least twice; ties go to the smaller root.
+__global__ __launch_bounds__(kSamples) void k_pick(int n, const int* parent, Ctrl* ctrl) {
+  __shared__ int s[kSamples];
+  __shared__ unsigned long long s_best[kSamples / 64];
+  const int t = threadIdx.x;
+  const int r = parent[(long long)t * n / kSamples];  // no union runs during this kernel
+  s[t] = r;
+  __syncthreads();
+  unsigned c = 0;
+  for (int k = 0; k < kSamples; k++) c += s[k] == r;
+  unsigned long long key = ((unsigned long long)c << 32) | (unsigned)(INT_MAX - r);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(key, d);
+    key = o > key ? o : key;
+  }
+  if (lane_id() == 0) s_best[t >> 6] = key;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long best = 0;
+    for (int k = 0; k < kSamples / 64; k++) best = s_best[k] > best ? s_best[k] : best;
+    ctrl->giant = (best >> 32) >= 2 ? INT_MAX - (int)(unsigned)best : -1;
+  }
+}
+
+// gbits[v] = (v's root is the largest tree's root), one ballot word per 64
+// vertices; with it the heavy phase drops the edges inside that tree after
+// two lookups in an array of n/8 bytes instead of gathering both roots.
+__global__ __launch_bounds__(kThreads) void k_members(int n, const int* parent, const Ctrl* ctrl,
+                                                      unsigned long long* gbits) {
+  const int R = ctrl->giant;
+  const long long stride = (long long)gridDim.x * kThreads;
+  const long long words = ((long long)n + 63) / 64;
+  for (long long base = (long long)blockIdx.x * kThreads; base < words * 64; base += stride) {
+    const long long v = base + threadIdx.x;
+    const unsigned long long b = __ballot(R >= 0 && v < n && parent[v] == R);
+    if (lane_id() == 0 && (v >> 6) < words) gbits[v >> 6] = b;
   }
 }
 
@@ -418,6 +477,7 @@ struct Layout {
   Ctrl* ctrl;
   int* parent;
   unsigned long long* minv;
+  unsigned long long* gbits;
   int4* wl[2];
   unsigned cap;
 };
@@ -426,6 +486,7 @@ size_t layout_bytes(long long n, long long m) {
   size_t b = 256;
   b += align_up(4 * (size_t)n, 256);
   b += align_up(8 * (size_t)n, 256);
+  b += align_up(8 * (size_t)((n + 63) / 64), 256);
   b += 2 * align_up(sizeof(int4) * (size_t)list_cap(m), 256);
   return b;
 }
@@ -439,6 +500,8 @@ Layout carve(void* base, long long n, long long m) {
   p += align_up(4 * (size_t)n, 256);
   L.minv = reinterpret_cast<unsigned long long*>(p);
   p += align_up(8 * (size_t)n, 256);
+  L.gbits = reinterpret_cast<unsigned long long*>(p);
+  p += align_up(8 * (size_t)((n + 63) / 64), 256);
   L.cap = list_cap(m);
   for (int k = 0; k < 2; k++) {
     L.wl[k] = reinterpret_cast<int4*>(p);
@@ -505,7 +568,8 @@ struct Runner {
       if (cnt > 0) {
         const int g = (int)std::min<long long>((cnt + kThreads - 1) / kThreads, (long long)blocks * 8);
         k_join<<<g, kThreads, 0, s>>>(L.wl[b], &L.ctrl->count[b], L.cap, L.parent, L.minv, in_mst);
-        k_reset<<<g, kThreads, 0, s>>>(L.wl[b], &L.ctrl->count[b], L.cap, L.minv);
+        const int gr = (int)std::min<long long>((n / 2 + kThreads - 1) / kThreads, (long long)blocks * 8);
+        k_reset<<<std::max(gr, 1), kThreads, 0, s>>>(n, L.minv);
       }
       a = b;
     }
@@ -518,7 +582,7 @@ struct Runner {
     const long long tiles = (m + kTile - 1) / kTile;
     const int nb = (int)std::max<long long>(1, std::min<long long>(blocks, tiles));
     const long long per = (tiles + nb - 1) / nb * kTile;
-    k_first<HEAVY><<<nb, kThreads, 0, s>>>(n, m, nindex, nlist, eweight, L.parent, L.ctrl, L.wl[0],
+    k_first<HEAVY><<<nb, kThreads, 0, s>>>(n, m, nindex, nlist, eweight, L.parent, L.gbits, L.ctrl, L.wl[0],
                                            L.cap, per);
     Ctrl c;
     int rc = read_ctrl(&c);
@@ -561,6 +625,9 @@ struct Runner {
     if (!rc && thr != INT_MAX) {
       const int gf = (int)std::min<long long>((n + kThreads - 1) / kThreads, (long long)blocks * 8);
       k_flatten<<<std::max(gf, 1), kThreads, 0, s>>>(n, L.parent);
+      k_pick<<<1, kSamples, 0, s>>>(n, L.parent, L.ctrl);
+      const int gm = (int)std::min<long long>(((n + 63) / 64 * 64 + kThreads - 1) / kThreads, (long long)blocks * 8);
+      k_members<<<std::max(gm, 1), kThreads, 0, s>>>(n, L.parent, L.ctrl, L.gbits);
       rc = phase<true>();
     }
     if (device_ms) {
@@ -600,7 +667,7 @@ int run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, const int3
   Ctrl* h = pinned_ctrl();
   if (!h) return fail(ECLMST_ERR_ALLOC, "could not allocate pinned memory");
   Runner r{nodes, edges, d_nindex, d_nlist, d_eweight, d_in_mst, carve(d_work, nodes, edges), s,
-           cu_count() * 4, h};
+           cu_count() * 8, h};
   if (edges == 0) {
     k_init<<<1, kThreads, 0, s>>>(nodes, r.L.parent, r.L.minv, 0, d_in_mst, r.L.ctrl);
     if (device_ms) *device_ms = 0.f;
