@@ -327,6 +327,32 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
   flush(A, wl, cnt, cap, flags);
 }
 
+// Proposes `key` to the minimum of `root` for the active lanes.  The lanes
+// whose root is the first active lane's (neighbouring list entries of one
+// vertex, and in later rounds the many edges into the largest tree) first
+// take their minimum across the wavefront and send one atomic for all of
+// them: every atomic on one address executes at the memory side, one after
+// the other.  All lanes of the wavefront call it.
+__device__ __forceinline__ void propose(unsigned long long* minv, int root, unsigned long long key,
+                                        bool active) {
+  const unsigned long long act = __ballot(active);
+  if (!act) return;
+  const int first = __ffsll((long long)act) - 1;
+  const int r0 = __shfl(root, first);
+  const bool same = active && root == r0;
+  unsigned long long k = same ? key : kNoEdge;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(k, d);
+    k = o < k ? o : k;
+  }
+  if (same) {
+    if (lane_id() == first) min64(&minv[r0], k);
+  } else if (active) {
+    min64(&minv[root], key);
+  }
+}
+
 // kernel1 (:240-256): keeps the listed edges whose ends are still in
 // different trees, as (root, root, w, j), and proposes each to both roots'
 // minima.  Persistent blocks over contiguous ranges of the input list.
@@ -356,11 +382,14 @@ __global__ __launch_bounds__(kThreads) void k_link(const int4* in, const unsigne
         if (e.x != e.y) {
           ok[r] = true;
           it[r] = e;
-          const unsigned long long key = key_of(e);
-          min64(&minv[e.x], key);
-          min64(&minv[e.y], key);
         }
       }
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+      const unsigned long long key = key_of(it[r]);
+      propose(minv, it[r].x, key, ok[r]);
+      propose(minv, it[r].y, key, ok[r]);
     }
     append(A, it, ok, out, out_cnt, cap, flags);
   }
