@@ -271,10 +271,10 @@ def racy(args) -> dict:
 
 def mst_bench(args) -> dict:
     """ECL-MST (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu) on the same
-    workload with main's default weights (:421-430) and threshold (:446-459).
+    workload with main's default weights (:428-437) and threshold (:450-462).
     One step = one forest, graph resident in HBM; the time is the device
     window from the first kernel to the last, which includes the per-round
-    reads of the worklist size, as the reference's GPUTimer does (:339-375).
+    reads of the worklist size, as the reference's GPUTimer does (:321-363).
     Checked at full size: the forest has n - c edges and the same components
     as the graph (c from the CC library); checked edge for edge against the
     oracle's Kruskal on R-MAT-21, which is also the CPU baseline."""
@@ -347,7 +347,7 @@ def mst_bench(args) -> dict:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"{g.name}, default weights (ECL-MST_10-atomic.cu:421-430)",
+        "config": {"workload": f"{g.name}, default weights (ECL-MST_10-atomic.cu:428-437)",
                    "nodes": g.nodes, "edges": g.edges, "threshold": thr,
                    "algorithmic_bytes": B},
         "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
@@ -362,7 +362,7 @@ def mst_bench(args) -> dict:
             "cores": 1,
             "kind": "port",
             "sample": f"{gs.name} (n={gs.nodes}, m={gs.edges}), the oracle's Kruskal "
-                      f"(ECL-MST_10-atomic.cu:113-154), median of {len(times)} runs",
+                      f"(ECL-MST_10-atomic.cu:112-154), median of {len(times)} runs",
             "seconds": statistics.median(times),
         },
     }

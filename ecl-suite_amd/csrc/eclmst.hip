@@ -2,9 +2,9 @@
 // (MI355X): kernels and the C ABI of include/eclmst.h (libeclmst.so).
 //
 // Reference algorithm: src/racefree/egr-input/MST/ECL-MST_10-atomic.cu
-//   find / join :157-174, initPM :177-185, initWL :188-237, kernel1 :240-256,
-//   kernel2 :259-271, kernel3 :274-282, host loop gpuMST :285-377, threshold
-//   choice in main :446-459.
+//   find / join :157-174, initPM :177-184, initWL :187-237, kernel1 :240-256,
+//   kernel2 :259-271, kernel3 :274-282, host loop gpuMST :285-382, threshold
+//   choice in main :450-462.
 //
 // Boruvka over an edge worklist, as the reference: every edge in the list
 // proposes itself, keyed weight << 32 | j, to the minimum of both of its
@@ -190,7 +190,7 @@ __device__ void append(Appender& A, const int4 (&it)[kPer], const bool (&ok)[kPe
 
 // ---- kernels -------------------------------------------------------------------
 
-// initPM (:177-185) and the zeroed output.
+// initPM (:177-184) and the zeroed output.
 __global__ __launch_bounds__(kThreads) void k_init(int n, int* parent, unsigned long long* minv,
                                                    long long m, unsigned char* in_mst,
                                                    Ctrl* ctrl) {
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void k_init(int n, int* parent, unsigned 
   }
 }
 
-// Threshold (main, :446-459): the sampled weight below which about 3 * n of
+// Threshold (main, :450-462): the sampled weight below which about 3 * n of
 // the m entries lie, from kSamples evenly spaced entries (rank by counting).
 __global__ __launch_bounds__(kSamples) void k_threshold(int n, long long m, const int* w,
                                                         Ctrl* ctrl) {
@@ -240,7 +240,7 @@ __device__ __forceinline__ int window_slot(const int* s, long long key) {
   return lo;
 }
 
-// initWL (:188-237), edge-parallel.  HEAVY = false: the entries i -> k with
+// initWL (:187-237), edge-parallel.  HEAVY = false: the entries i -> k with
 // k > i and weight <= threshold, as (i, k, w, j).  HEAVY = true (after
 // k_flatten): those with weight > threshold whose ends lie in different
 // trees, as (root i, root k, w, j).  Every weight is checked for sign.

@@ -2,8 +2,8 @@
 
 Mirror of the reference's host interface for the MST path
 (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu): ``compute_mst`` is
-``gpuMST`` with main's threshold choice (:285-377, :446-459), ``mst_weights``
-is main's default weights (:421-430).  The output is the reference's
+``gpuMST`` with main's threshold choice (:285-382, :450-462), ``mst_weights``
+is main's default weights (:428-437).  The output is the reference's
 ``inMST`` array: one byte per CSR entry, 1 for the entry j of vertex i with
 nlist[j] > i of every forest edge.  No CPU fallback: calling without the
 built library raises NativeLibraryMissing.

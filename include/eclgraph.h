@@ -48,7 +48,7 @@ int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const in
                    int flags, int64_t* where);
 
 /* Default edge weights of the reference's ECL-MST main
-   (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu:421-430): the entry j of
+   (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu:428-437): the entry j of
    vertex i with neighbour k weighs 1 + ((i * k) % nodes), negated if
    negative, with i * k wrapping in 32-bit two's complement as the
    reference's int product does.  eweight gets nindex[nodes] entries. */

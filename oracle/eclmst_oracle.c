@@ -11,7 +11,7 @@
 #include <string.h>
 
 /* Default weights of the reference's main
-   (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu:421-430): the entry j of
+   (src/racefree/egr-input/MST/ECL-MST_10-atomic.cu:428-437): the entry j of
    vertex i with neighbour k weighs 1 + ((i * k) % nodes), made positive if
    negative.  i * k is an int product there, so it wraps in two's
    complement; the product is formed in unsigned arithmetic here to get the
@@ -46,7 +46,7 @@ static int32_t mst_find(int32_t* parent, int32_t x) {
   return x;
 }
 
-/* Kruskal (ECL-MST_10-atomic.cu:113-154): the edges i-k with k > i (one
+/* Kruskal (ECL-MST_10-atomic.cu:112-154): the edges i-k with k > i (one
    direction, entry j) sorted by (weight, j), each joined if its ends are in
    different trees.  Weights must be non-negative (the reference's are >= 0
    by construction); the key keeps the weight in the high 32 bits. */

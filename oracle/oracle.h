@@ -95,8 +95,8 @@ int64_t oracle_count_components(int32_t n, const int32_t* label);
    k = nlist[j] > i) of every edge of the minimum spanning forest under the
    strict order (weight, j); 0 for every other entry, including the k < i
    direction.  The reference's GPU code keys each edge by weight << 32 | j
-   (:249-251) and its CPU Kruskal sorts by the tuple (weight, j, i, k)
-   (:121-131), so both pick this one forest.
+   (:251) and its CPU Kruskal sorts by the tuple (weight, j, i, k)
+   (:122-131), so both pick this one forest.
    Pinned (tests/test_mst.py): the forest's total weight equals
    scipy.sparse.csgraph.minimum_spanning_tree's on random graphs; the edge
    set equals the brute-force minimum over all spanning forests of the keys
@@ -104,10 +104,10 @@ int64_t oracle_count_components(int32_t n, const int32_t* label);
    n - (number of components) edges and joins each component; closed forms
    for a tree (every edge) and for a cycle (all but the largest key). */
 
-/* Default weights (ECL-MST_10-atomic.cu:421-430). */
+/* Default weights (ECL-MST_10-atomic.cu:428-437). */
 void oracle_mst_weights(int32_t n, const int32_t* nindex, const int32_t* nlist, int32_t* w);
 
-/* Kruskal (ECL-MST_10-atomic.cu:113-154).  Weights must be >= 0.  Returns 0,
+/* Kruskal (ECL-MST_10-atomic.cu:112-154).  Weights must be >= 0.  Returns 0,
    -1 on bad arguments, -2 on an ID out of range or a negative weight, -3 if
    out of memory. */
 int oracle_mst(int32_t n, const int32_t* nindex, const int32_t* nlist, const int32_t* w,

@@ -119,7 +119,7 @@ def count_components(labels) -> int:
 
 
 def mst_weights(nindex, nlist) -> np.ndarray:
-    """The reference's default edge weights (ECL-MST_10-atomic.cu:421-430)."""
+    """The reference's default edge weights (ECL-MST_10-atomic.cu:428-437)."""
     ni, nl, n = _csr(nindex, nlist)
     w = np.empty(nl.shape[0], dtype=np.int32)
     lib().oracle_mst_weights(n, ni.ctypes.data, nl.ctypes.data if nl.size else None,
@@ -129,7 +129,7 @@ def mst_weights(nindex, nlist) -> np.ndarray:
 
 def mst(nindex, nlist, weights) -> np.ndarray:
     """in_mst[j] (uint8) of the minimum spanning forest under (weight, j), by
-    Kruskal (ECL-MST_10-atomic.cu:113-154)."""
+    Kruskal (ECL-MST_10-atomic.cu:112-154)."""
     ni, nl, n = _csr(nindex, nlist)
     w = np.ascontiguousarray(weights, dtype=np.int32)
     out = np.empty(max(nl.shape[0], 1), dtype=np.uint8)
