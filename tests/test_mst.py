@@ -219,3 +219,37 @@ def test_gpu_mst_rejects_bad_inputs(cuda):
     with pytest.raises(N.EclmstError) as e:
         mst.compute_mst(gd, np.ones(nlist.shape[0], dtype=np.int32))
     assert e.value.status == -6
+
+
+def test_mst_cli_usage_and_file_errors(tmp_path):
+    import subprocess
+
+    exe = N.BIN_DIR / "eclmst"
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 255 and "USAGE:" in r.stderr
+    r = subprocess.run([str(exe), str(tmp_path / "none.egr")], capture_output=True, text=True)
+    assert r.returncode == 255 and "ERROR:" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("weighted", [False, True])
+def test_gpu_mst_cli_matches_the_harness_protocol(cuda, tmp_path, weighted):
+    # the reference's main prints "Dev+/-: %12.9f s" and "all good"; its
+    # analyzer turns Dev[-+] into "runtime" and reads `time: +(\d+\.\d+) s`
+    # (scripts/analyze_results.py:62,83-84)
+    import re
+    import subprocess
+
+    g = workloads.rmat(12, 8, 6) if weighted else Graph.er(2000, 3000, 4)
+    if weighted:
+        g.eweight = (mst.mst_weights(g) % 50).astype(np.int32)
+    path = tmp_path / "g.egr"
+    g.write(str(path))
+    announce(f"eclmst CLI on {g.nodes} nodes, weighted file: {weighted}")
+    r = subprocess.run([str(N.BIN_DIR / "eclmst"), str(path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "all good" in r.stdout and "differ" not in r.stdout
+    assert ("Dev+:" if g.edges // g.nodes >= 4 else "Dev-:") in r.stdout
+    line = next(l for l in r.stdout.splitlines() if l.startswith("Dev"))
+    assert re.search(r"time: +(\d+\.\d+) s", re.sub(r"Dev[-+]", "runtime", line))
