@@ -253,3 +253,33 @@ def test_gpu_mst_cli_matches_the_harness_protocol(cuda, tmp_path, weighted):
     assert ("Dev+:" if g.edges // g.nodes >= 4 else "Dev-:") in r.stdout
     line = next(l for l in r.stdout.splitlines() if l.startswith("Dev"))
     assert re.search(r"time: +(\d+\.\d+) s", re.sub(r"Dev[-+]", "runtime", line))
+
+
+def test_mst_shared_accesses_go_through_atomic_helpers():
+    """parent[] and minv[] are touched through the relaxed agent-scope
+    helpers everywhere two threads of one kernel may meet on them; the plain
+    accesses left are in kernels that only write the array at distinct
+    addresses (k_init, k_reset) or only read it (k_pick, k_members)."""
+    import re
+
+    src = (N.PKG_ROOT / "csrc" / "eclmst.hip").read_text()
+    body = re.sub(r"//[^\n]*", "", src)
+    helpers = r"(?:ld|st|cas|ld64|st64|min64)\(&$"
+    plain = []
+    for m in re.finditer(r"\b(parent|minv)\[", body):
+        if re.search(helpers, body[max(0, m.start() - 10):m.start()]):
+            continue
+        k = body.rfind("__global__", 0, m.start())
+        kernel = re.search(r"void (\w+)\(", body[k:k + 300]).group(1)
+        plain.append((kernel, m.group(1)))
+    assert sorted(set(plain)) == [("k_init", "minv"), ("k_init", "parent"), ("k_members", "parent"),
+                                  ("k_pick", "parent"), ("k_reset", "minv")], plain
+    for kernel in ("k_init", "k_reset", "k_pick", "k_members"):
+        k = body.find(f"void {kernel}(")
+        kbody = body[k:body.find("\n}\n", k)]
+        assert not re.search(r"\b(ld|st|cas|ld64|st64|min64)\(&(parent|minv)", kbody), kernel
+    prod = src.split("// ---- atomics", 1)[1].split("// Root of x", 1)[0]
+    for f in ("__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)",
+              "__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)",
+              "__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)"):
+        assert f in prod, f
