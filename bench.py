@@ -351,6 +351,16 @@ def mst_bench(args) -> dict:
                    "nodes": g.nodes, "edges": g.edges, "threshold": thr,
                    "algorithmic_bytes": B},
         "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
+        "pass_roofline": {
+            "bound": "hbm",
+            "achieved": B / (ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": B / (ms * 1e-3) / HBM_PEAK,
+            "traffic": None,
+            "algorithmic_bytes": B,
+            "bytes_rule": "4(n+1) + 4m + 4m + m: nindex, nlist and weights read once, in_mst written",
+        },
         "forest_edges": int(sel.sum()),
         "forest_weight": weight,
         "components": ncomp,
