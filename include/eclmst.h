@@ -22,7 +22,9 @@
  *   by (weight, j, ...) (:122-131).
  *
  * Conventions as in eclcc.h: 0 or a negative ECLMST_ERR_* status, never an
- * exit; eclmst_last_error() gives the message (per host thread).
+ * exit; eclmst_last_error() gives the message (per host thread).  Calls
+ * from several host threads are safe and run one at a time (they share the
+ * small pinned buffer through which each round's list size is read).
  */
 #ifndef ECLMST_ABI_H
 #define ECLMST_ABI_H
