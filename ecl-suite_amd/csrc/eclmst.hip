@@ -39,7 +39,6 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
-#include <cstring>
 #include <mutex>
 #include <string>
 
@@ -547,7 +546,8 @@ int cu_count() {
   return cus;
 }
 
-// Small pinned host buffer for the per-round control reads.
+// Small pinned host buffer for the per-round control reads, allocated on
+// first use and held until the process ends.
 Ctrl* pinned_ctrl() {
   static Ctrl* h = nullptr;
   static std::once_flag once;
