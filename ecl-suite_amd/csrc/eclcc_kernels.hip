@@ -23,7 +23,10 @@
 //   compute3  high degree: block granularity, edge tasks staged through LDS
 //   flatten   labels := root (reference flatten)
 // With filter 2 hubscan and compute3 are not launched (mark hands the few
-// high-degree vertices left to compute2).
+// high-degree vertices left to compute2).  A batch runs the same sequence
+// over the slots of all its graphs (BATCH = true): sample and mark then
+// work per block list and per 256-slot chunk, with their blocks taken in a
+// spread order over the batch (spread_block).
 //
 // Why the bitmap (MI355X measurements): random 4-byte gathers from the
 // 64 MB label array of a 2^24-vertex graph run at ~58 G/s (memory side),
