@@ -64,6 +64,8 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   t.filter = p.giant_filter;
   if (p.path_mode != 0 && p.path_mode != 1) return set_error(ECLCC_ERR_ARG, "path_mode must be 0 or 1");
   t.path_mode = p.path_mode;
+  if (p.sample_shift < 0 || p.sample_shift > 4) return set_error(ECLCC_ERR_ARG, "sample_shift must be in [0, 4]");
+  t.sample_shift = p.sample_shift;
   if (t.thread_max < 0 || t.wave_max < t.thread_max)
     return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
   if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
@@ -145,6 +147,7 @@ void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
   p->hub_chunk = t.hub_chunk;
   p->filter = t.filter;
   p->path_mode = t.path_mode;
+  p->sample_shift = t.sample_shift;
 }
 
 }  // namespace eclcc
@@ -617,6 +620,7 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   prm.block_threads = b->tuning.block_threads;
   prm.giant_filter = b->tuning.filter;
   prm.path_mode = b->tuning.path_mode;
+  prm.sample_shift = b->tuning.sample_shift;
   return eclcc_run_device(d.n, d.m, d.nidx, d.nlist, d.nstat, b->d_work1, b->work1_bytes, &prm,
                           stream);
 }

@@ -79,6 +79,7 @@ struct KParams {
   int filter;     // 0: off; 1: skip edges with both ends in the sampled largest
                   // component; 2: also leave the lists of its members unread
   int path_mode;  // 0: pointer jumping at every step (reference); 1: root jump
+  int sample_shift;  // second-entry hooks taken by 1 in 2^sample_shift vertices
 };
 
 // Workspace carved out of one device buffer.
@@ -106,7 +107,7 @@ struct Shape {
 
 // Resolved, validated parameters.
 struct Tuning {
-  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter, path_mode;
+  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter, path_mode, sample_shift;
 };
 
 size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs);

@@ -37,11 +37,11 @@ class EclccParams(C.Structure):
         ("block_threads", i32),
         ("giant_filter", i32),
         ("path_mode", i32),
-        ("reserved", i32 * 1),
+        ("sample_shift", i32),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class EclccServerConfig(C.Structure):

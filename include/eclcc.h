@@ -73,7 +73,11 @@ typedef struct eclcc_params {
                              start vertex, to the root (one store per walk).  The
                              sampled hooks walk both paths together to where they meet
                              in either mode */
-  int32_t reserved[1];
+  int32_t sample_shift;   /* the sampled hook of a vertex that has a smaller neighbour
+                             (its second entry) is taken by 1 in 2^sample_shift of
+                             them, picked by a hash of the ID; 0 (default): by all.
+                             Vertices without a smaller neighbour always take theirs.
+                             The labels do not depend on it (0..4) */
 } eclcc_params;
 
 /* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */

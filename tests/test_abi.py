@@ -39,8 +39,12 @@ def test_params_defaults_and_workspace():
     assert p.giant_filter == 2 and p.path_mode == 1
     for f in (0, 1, 2):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(giant_filter=f))) > 0
+    assert p.sample_shift == 0
+    for sh in range(5):
+        assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(sample_shift=sh))) > 0
     for bad in (eclcc.default_params(giant_filter=3), eclcc.default_params(giant_filter=-1),
-                eclcc.default_params(path_mode=2)):
+                eclcc.default_params(path_mode=2), eclcc.default_params(sample_shift=5),
+                eclcc.default_params(sample_shift=-1)):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
 
 

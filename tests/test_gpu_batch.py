@@ -51,7 +51,7 @@ def test_batch_with_tuned_params(cuda, filt):
     gs = small_mix(10, seed=5, n_hi=60000) + [workloads.grid3d(20, 0.2, 3), workloads.rmat(12, 2, 4)]
     announce(f"batch with non-default params, giant_filter={filt}")
     p = default_params(thread_max_deg=2, wave_max_deg=32, hub_chunk=256, block_threads=512,
-                       giant_filter=filt)
+                       giant_filter=filt, sample_shift=filt)
     expect = [oracle.cc_labels(g.nindex, g.nlist) for g in gs]
     with Batch(gs, p) as b:
         for rep in range(2):

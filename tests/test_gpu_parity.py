@@ -72,6 +72,10 @@ PARAMS = [
     dict(giant_filter=2, thread_max_deg=2, wave_max_deg=40, hub_chunk=128, path_mode=0),
     dict(giant_filter=2, thread_max_deg=1 << 20, wave_max_deg=1 << 20),
     dict(giant_filter=2, thread_max_deg=16, wave_max_deg=704, block_threads=1024, hub_blocks=3),
+    dict(sample_shift=1),
+    dict(sample_shift=4),
+    dict(sample_shift=2, giant_filter=1),
+    dict(sample_shift=3, giant_filter=0, path_mode=0),
 ]
 
 
