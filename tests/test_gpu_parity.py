@@ -228,3 +228,28 @@ def test_high_degree_vertices_outside_the_largest_tree(cuda, filt):
         announce(f"giant_filter={filt}: {name}")
         for kw in (dict(), dict(thread_max_deg=0, wave_max_deg=1), dict(wave_max_deg=100)):
             check(g, compute(g, default_params(giant_filter=filt, **kw)))
+
+
+def test_cli_runs_like_the_reference_minibench(cuda, tmp_path):
+    # the process face: `eclcc <file.egr>` prints the reference's lines, and
+    # its analyzer reads the time with `time: +(\d+\.\d+) s`
+    # (scripts/analyze_results.py:62)
+    import re
+    import subprocess
+
+    from eclcc import _native as N
+
+    g = G.disjoint_cliques(30, 9, shuffle_seed=4)
+    g2 = workloads.rmat(13, 8, 2)
+    for graph, flags in ((g, []), (g2, ["--check"])):
+        path = tmp_path / "g.egr"
+        graph.write(str(path))
+        announce(f"eclcc CLI {flags} on n={graph.nodes}")
+        r = subprocess.run([str(N.BIN_DIR / "eclcc"), *flags, str(path)], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = r.stdout
+        assert "verification passed" in out
+        assert re.search(r"time: +(\d+\.\d+) s", out)
+        want = oracle.count_components(oracle.cc_labels(graph.nindex, graph.nlist))
+        assert f"number of connected components: {want}\n" in out
