@@ -351,34 +351,37 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   if (live) {
     beg = G.nidx[v];
     end = G.nidx[v + 1];
-    x0 = beg + 0 < end ? G.nlist[beg + 0] : INT_MAX;
-    x1 = beg + 1 < end ? G.nlist[beg + 1] : INT_MAX;
+    x0 = end - beg > 0 ? G.nlist[beg + 0] : INT_MAX;
+    x1 = end - beg > 1 ? G.nlist[beg + 1] : INT_MAX;
     m = x0 < v ? x0 : x1 < v ? x1 : v;
   }
-  int pos = beg + 2;
-  bool scan = live && m == v && pos < end;
+  // list positions are unsigned: a list may end at entry 2^31 - 1, and
+  // the positions one step past it must not wrap
+  const unsigned uend = (unsigned)end;
+  unsigned pos = (unsigned)beg + 2u;
+  bool scan = live && m == v && pos < uend;
   while (__ballot(scan)) {
     if (scan) {
       int x[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) x[k] = pos + k < end ? G.nlist[pos + k] : INT_MAX;
+      for (int k = 0; k < 8; k++) x[k] = pos + k < uend ? G.nlist[pos + k] : INT_MAX;
 #pragma unroll
       for (int k = 0; k < 8; k++) m = (m == v && x[k] < v) ? x[k] : m;
       pos += 8;
-      scan = m == v && pos < end && pos < beg + kLaneScan;
+      scan = m == v && pos < uend && pos < (unsigned)beg + kLaneScan;
     }
   }
-  const bool longscan = live && m == v && pos < end;
+  const bool longscan = live && m == v && pos < uend;
   unsigned long long pending = __ballot(longscan);
   while (pending) {
     const int src = __ffsll((long long)pending) - 1;
     pending &= pending - 1;
     const int sv = __shfl(v, src);
-    const int sb = __shfl(pos, src);
-    const int se = __shfl(end, src);
+    const unsigned sb = __shfl(pos, src);
+    const unsigned se = __shfl(uend, src);
     int found = sv;
-    for (int base = sb; base < se; base += 64) {
-      const int i = base + lane;
+    for (unsigned base = sb; base < se; base += 64) {
+      const unsigned i = base + lane;
       const int x = i < se ? G.nlist[i] : INT_MAX;
       const unsigned long long hit = __ballot(x < sv);
       if (hit) {
@@ -520,6 +523,12 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride
 //    with both ends in the tree is hooked by nobody: its ends are already
 //    in one tree.  An edge with one end outside is hooked from that end.
 
+// Tasks of hub_chunk entries that cover a list of deg entries (unsigned, so
+// that a list of nearly 2^31 entries does not wrap).
+__device__ __forceinline__ unsigned task_count(int deg, int chunk) {
+  return ((unsigned)deg + (unsigned)chunk - 1u) / (unsigned)chunk;
+}
+
 // Sum of blocktot[0, nb) over the threads of a block (packed hub and task
 // counts add without carrying into each other).  All threads call it.
 __device__ __forceinline__ unsigned long long sum_block_totals(const KParams& p, long long nb) {
@@ -553,7 +562,7 @@ __device__ __forceinline__ void emit_hubs(const KParams& p, long long w, unsigne
   if (hub) {
     hbeg = G.nidx[v];
     hend = G.nidx[v + 1];
-    tasks = (unsigned)((hend - hbeg + p.hub_chunk - 1) / p.hub_chunk);
+    tasks = task_count(hend - hbeg, p.hub_chunk);
   }
   const unsigned excl = wave_incl_sum(tasks) - tasks;
   if (hub) {
@@ -584,10 +593,10 @@ __device__ __forceinline__ void hook_low(const KParams& p, long long slot) {
     const int s = lab_ld(&ns[v]);
     int vstat = -1;  // found on first use
     bool joined = false;
-    for (int i = beg; i < end; i += 8) {
+    for (unsigned i = beg; i < (unsigned)end; i += 8) {
       int u[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : -1;
+      for (int k = 0; k < 8; k++) u[k] = i + k < (unsigned)end ? ld_stream(&nl[i + k]) : -1;
       bool need[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) need[k] = u[k] >= 0 && u[k] != v;
@@ -614,10 +623,10 @@ __device__ __forceinline__ void hook_low(const KParams& p, long long slot) {
   }
   const bool marked = p.filter && slot_bit(p.gbits, slot);
   int vstat = marked ? R : -1;  // found on first use otherwise
-  for (int i = beg; i < end; i += 8) {
+  for (unsigned i = beg; i < (unsigned)end; i += 8) {
     int u[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) u[k] = i + k < end ? ld_stream(&nl[i + k]) : INT_MAX;
+    for (int k = 0; k < 8; k++) u[k] = i + k < (unsigned)end ? ld_stream(&nl[i + k]) : INT_MAX;
     bool need[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) need[k] = v > u[k];
@@ -803,7 +812,7 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
     unsigned tasks = 0;
     if ((hb >> lane) & 1ull) {
       const int deg = G.nidx[v + 1] - G.nidx[v];
-      tasks = (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk);
+      tasks = task_count(deg, p.hub_chunk);
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
@@ -890,7 +899,7 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
       unsigned tasks = 0;
       if ((hb >> lane) & 1ull) {
         const int deg = G.nidx[v[k] + 1] - G.nidx[v[k]];
-        tasks = (unsigned)((deg + p.hub_chunk - 1) / p.hub_chunk);
+        tasks = task_count(deg, p.hub_chunk);
       }
 #pragma unroll
       for (int d = 32; d > 0; d >>= 1) tasks += __shfl_xor(tasks, d);
@@ -1010,24 +1019,27 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
     s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
     s_root[wv][lane] = root;
     s_vtx[wv][lane] = v;
-    for (int w0 = 0; w0 < total; w0 += kWaveWindow) {
-      const int wend = w0 + kWaveWindow < total ? w0 + kWaveWindow : total;
+    // window positions are unsigned: the lists of one wavefront may hold
+    // nearly 2^31 entries, and a step past the last must not wrap
+    const unsigned utotal = (unsigned)total;
+    for (unsigned w0 = 0; w0 < utotal; w0 += kWaveWindow) {
+      const unsigned wend = w0 + kWaveWindow < utotal ? w0 + kWaveWindow : utotal;
       for (unsigned long long m = bits; m; m &= m - 1) {
         const int L = __ffsll((long long)m) - 1;
         const int oL = __builtin_amdgcn_readlane(off, L);
         const int dL = __builtin_amdgcn_readlane(deg, L);
-        const int a = oL > w0 ? oL : w0;
-        const int b = oL + dL < wend ? oL + dL : wend;
-        for (int x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
+        const unsigned a = (unsigned)oL > w0 ? (unsigned)oL : w0;
+        const unsigned b = (unsigned)(oL + dL) < wend ? (unsigned)(oL + dL) : wend;
+        for (unsigned x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int e0 = w0; e0 < wend; e0 += 256) {
+      for (unsigned e0 = w0; e0 < wend; e0 += 256) {
         int u[4], src[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const int e = e0 + k * 64 + lane;
+          const unsigned e = e0 + k * 64 + lane;
           const bool valid = e < wend;
           src[k] = valid ? owner[e - w0] : 0;
           u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
@@ -1177,7 +1189,7 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
     const int cend = (int)(cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end);
     // root estimate of v: the sampled root if v is marked, else found once
     const int vstat = vin ? Rg : find_root(v, G.nstat, p.path_mode);
-    for (int base = cbeg; base < cend; base += R * BT) {
+    for (unsigned base = cbeg; base < (unsigned)cend; base += R * BT) {
       __syncthreads();
       if (s_cnt > CAP - R * BT) {
         const int cnt = s_cnt;
@@ -1189,8 +1201,8 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
       int u[R];
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int i = base + r * BT + tid;
-        u[r] = i < cend ? ld_stream(&G.nlist[i]) : INT_MAX;
+        const unsigned i = base + r * BT + tid;
+        u[r] = i < (unsigned)cend ? ld_stream(&G.nlist[i]) : INT_MAX;
       }
       unsigned keep = 0;
       if (skip) {

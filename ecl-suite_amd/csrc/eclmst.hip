@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
   long long t0 = jb0;
   int iw = s_i0;
   while (t0 < jb1) {
-    for (int x = threadIdx.x; x <= kWin; x += kThreads) s_idx[x] = nindex[std::min(iw + x, n)];
+    for (int x = threadIdx.x; x <= kWin; x += kThreads) s_idx[x] = nindex[std::min<long long>((long long)iw + x, n)];
     __syncthreads();
     const long long te = std::max<long long>(t0, std::min<long long>(std::min<long long>(t0 + kTile, jb1), s_idx[kWin]));
     if (te > t0) {
@@ -318,10 +318,10 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
       if (neg) __hip_atomic_fetch_or(flags, kFlagNegative, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       append(A, it, ok, wl, cnt, cap, flags);
     }
-    const int niw = te < s_idx[kWin] ? iw + window_slot(s_idx, te) : iw + kWin;
+    const long long niw = (long long)iw + (te < s_idx[kWin] ? window_slot(s_idx, te) : kWin);
     __syncthreads();
     t0 = te;
-    iw = std::min(niw, n);
+    iw = (int)std::min<long long>(niw, n);
   }
   flush(A, wl, cnt, cap, flags);
 }

@@ -90,6 +90,18 @@ class DeviceGraph:
         self.work_bytes = int(wb)
         torch.cuda.synchronize(self.device)
 
+    def set_params(self, params: N.EclccParams) -> None:
+        """Switches to other thresholds and modes for later runs; the CSR
+        stays where it is, and the workspace grows if they need more."""
+        wb = N.cc_lib().eclcc_workspace_bytes(self.nodes, self.edges, C.byref(params))
+        if wb == 0:
+            raise N.EclccError(-1, "bad graph size or parameters for the workspace")
+        if wb > self.work_bytes:
+            self.torch.cuda.synchronize(self.device)
+            self.work = self.torch.empty(int(wb), dtype=self.torch.uint8, device=self.device)
+            self.work_bytes = int(wb)
+        self.params = params
+
     def _stream(self, stream) -> Optional[int]:
         if stream is None:
             stream = self.torch.cuda.current_stream(self.device)

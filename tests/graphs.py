@@ -70,3 +70,61 @@ def two_stars_bridged(n_leaves, hub_a, hub_b, n):
     chain = [hub_a] + rest[:50] + [hub_b]
     pairs += [(chain[i], chain[i + 1]) for i in range(len(chain) - 1)]
     return from_undirected(n, pairs, "two-stars")
+
+
+INT32_MAX = 2**31 - 1
+
+
+def max_entries_multigraph(chain=40, split=20, tail_loops=101, x=30, edges=INT32_MAX):
+    """A symmetric multigraph with `edges` CSR entries (default 2^31 - 1, the
+    most an int32 edge count allows), built so that lists end right at that
+    limit.  Vertices 0..chain-1 form two paths, 0..split-1 and split..chain-1;
+    each path edge is repeated c times (the first edge of each path once, so
+    that the two path roots have short lists, and the last edge takes the
+    rest of the count).  Vertex z = chain has `tail_loops` self-loops and then
+    one edge to x (x's list ends with z): z has no smaller neighbour among
+    its first entries, so init scans its list up to the final entry.
+    Vertex z + 1 is isolated, with its empty list at position `edges`.
+    Labels: 0 on the first path, split on the second and on z, z + 1 on z + 1.
+    Returns (graph, expected labels)."""
+    assert 2 < split < chain - 2 and split <= x < chain
+    z = chain
+    base = 2 + 2 + 2 + tail_loops  # edges 0-1 and split-(split+1) once, z-x, z's loops
+    rest = edges - base
+    assert rest > 0 and rest % 2 == 0, "edges - tail_loops must be odd"
+    many = [i for i in range(chain - 1) if i not in (0, split - 1, split, chain - 2)]
+    c = rest // 2 // (len(many) + 1)
+    c_last = rest // 2 - c * len(many)
+    assert c >= 1 and c_last >= 1
+
+    def mult(i):  # multiplicity of path edge (i, i + 1)
+        if i == split - 1:
+            return 0
+        if i in (0, split):
+            return 1
+        return c_last if i == chain - 2 else c
+
+    vert, neigh, cnt = [], [], []
+    for v in range(chain):
+        for u, k in ((v - 1, mult(v - 1) if v > 0 else 0), (v + 1, mult(v) if v < chain - 1 else 0)):
+            if k:
+                vert.append(v), neigh.append(u), cnt.append(k)
+        if v == x:
+            vert.append(v), neigh.append(z), cnt.append(1)
+    vert += [z, z]
+    neigh += [z, x]
+    cnt += [tail_loops, 1]
+    cnt = np.asarray(cnt, dtype=np.int64)
+    n = chain + 2
+    deg = np.bincount(np.asarray(vert), weights=cnt, minlength=n).astype(np.int64)
+    nindex = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(deg, out=nindex[1:])
+    assert nindex[-1] == edges
+    nlist = np.repeat(np.asarray(neigh, dtype=np.int32), cnt)
+    g = Graph(nindex.astype(np.int32), nlist, name=f"multigraph-m{edges}")
+    expected = np.empty(n, dtype=np.int32)
+    expected[:split] = 0
+    expected[split:chain] = split
+    expected[z] = split
+    expected[z + 1] = z + 1
+    return g, expected

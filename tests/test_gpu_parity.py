@@ -253,3 +253,36 @@ def test_cli_runs_like_the_reference_minibench(cuda, tmp_path):
         assert re.search(r"time: +(\d+\.\d+) s", out)
         want = oracle.count_components(oracle.cc_labels(graph.nindex, graph.nlist))
         assert f"number of connected components: {want}\n" in out
+
+
+def test_lists_ending_at_the_largest_int32_edge_count(cuda):
+    # m = 2^31 - 1 CSR entries (8 GiB of nlist), the most an int32 edge
+    # count allows: the last non-empty list ends at entry 2^31 - 2 and is
+    # scanned by init to its end, then hooked at lane (compute1), wavefront
+    # (compute2) and block (compute3) granularity; the last vertex's empty
+    # list starts at 2^31 - 1.  List positions one step past the end must
+    # not wrap.  Labels by construction (two paths of repeated edges, a
+    # vertex joined to the second path, an isolated vertex).
+    import torch
+
+    g, expected = G.max_entries_multigraph()
+    assert g.edges == 2**31 - 1 and int(g.nindex[-2]) == 2**31 - 1
+    announce(f"plain eclcc_compute host arrays {g.name} n={g.nodes}")
+    assert np.array_equal(compute(g), expected)
+    dg = DeviceGraph(g, "cuda:0")
+    try:
+        for kw in (dict(), dict(giant_filter=0), dict(giant_filter=1, path_mode=0),
+                   dict(giant_filter=0, thread_max_deg=128, wave_max_deg=256),
+                   dict(giant_filter=2, thread_max_deg=128, wave_max_deg=256),
+                   dict(giant_filter=0, thread_max_deg=16, wave_max_deg=64),
+                   dict(giant_filter=2, thread_max_deg=16, wave_max_deg=64, hub_chunk=64)):
+            announce(f"resident {g.name} params {kw}")
+            dg.set_params(default_params(**kw))
+            dg.nstat.fill_(-7)
+            dg.run()
+            got = dg.labels()
+            bad = np.nonzero(got != expected)[0]
+            assert bad.size == 0, f"{kw}: {bad.size} labels differ, first at {bad[:5]}: {got[bad[:5]]}"
+    finally:
+        del dg
+        torch.cuda.empty_cache()

@@ -130,3 +130,17 @@ def test_verifier_catches_planted_mistakes():
     bad = lab.copy()
     bad[3] = len(lab)
     assert oracle.verify(ni, nl, bad)[0] == -6
+
+
+def test_max_entries_multigraph_builder_small():
+    # the builder of the 2^31 - 1 entry GPU case, at a size the oracle
+    # checks in milliseconds: symmetric, lists where stated, labels as stated
+    g, exp = G.max_entries_multigraph(chain=12, split=5, tail_loops=11, x=7, edges=10001)
+    assert g.edges == 10001 and int(g.nindex[-2]) == 10001
+    src = np.repeat(np.arange(g.nodes, dtype=np.int64), np.diff(g.nindex))
+    fwd = np.sort(src * g.nodes + g.nlist)
+    rev = np.sort(g.nlist.astype(np.int64) * g.nodes + src)
+    assert np.array_equal(fwd, rev)
+    assert np.array_equal(oracle.cc_labels(g.nindex, g.nlist), exp)
+    z = 12
+    assert list(g.nlist[g.nindex[z]:g.nindex[z + 1]]) == [z] * 11 + [7]
