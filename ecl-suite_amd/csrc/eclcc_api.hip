@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -27,6 +28,21 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 int set_error(int code, const std::string& msg) {
   t_err = msg;
   return code;
+}
+
+// Host exceptions (an allocation that fails, a thread that cannot start)
+// never cross the C ABI: every entry point that can meet one turns it into
+// a status here.
+int host_exception() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return set_error(ECLCC_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception& e) {
+    return set_error(ECLCC_ERR_ALLOC, std::string("host resources: ") + e.what());
+  } catch (...) {
+    return set_error(ECLCC_ERR_ALLOC, "host error");
+  }
 }
 
 int check_hip(hipError_t e, int code, const char* what) {
@@ -256,15 +272,17 @@ void eclcc_params_default(eclcc_params* p) {
   p->path_mode = 1;
 }
 
-size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) {
+size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) try {
   Tuning t;
   if (nodes < 1 || edges < 0 || resolve_tuning(params, &t) != ECLCC_OK) return 0;
   return workspace_bytes(nodes, hub_capacity(edges, t.wave_max, 1), 1);
+} catch (...) {
+  return (host_exception(), size_t(0));
 }
 
 int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, const int32_t* d_nlist,
                      int32_t* d_nstat, void* d_work, size_t work_bytes, const eclcc_params* params,
-                     void* stream) {
+                     void* stream) try {
   int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
   if (rc) return rc;
   if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
@@ -277,12 +295,14 @@ int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, cons
   return check_hip(launch_pipeline(p, false, Shape{t.hub_blocks, t.block_threads},
                                    (hipStream_t)stream, nullptr),
                    ECLCC_ERR_LAUNCH, "kernel launch failed");
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                               const int32_t* d_nlist, int32_t* d_nstat, void* d_work,
                               size_t work_bytes, const eclcc_params* params, void* stream,
-                              float* kernel_ms, float* total_ms) {
+                              float* kernel_ms, float* total_ms) try {
   int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
   if (rc) return rc;
   if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
@@ -316,11 +336,13 @@ int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nin
   for (int i = 0; i <= kNumKernels; i++) (void)hipEventDestroy(ev[i]);
   if (!rc) rc = overflow_check(w, s);
   return rc;
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                         const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
-                        void* stream) {
+                        void* stream) try {
   int rc = check_csr_args(nodes, edges, d_nindex, d_nlist);
   if (rc) return rc;
   if (!d_nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
@@ -332,6 +354,8 @@ int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
   const KParams p = single_params(nodes, d_nindex, d_nlist, d_nstat, w, t);
   return check_hip(launch_init_only(p, (hipStream_t)stream), ECLCC_ERR_LAUNCH,
                    "kernel launch failed");
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, int64_t* n_hubs,
@@ -349,7 +373,7 @@ int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, i
 }
 
 int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
-                        int32_t* nstat, const eclcc_params* params, double* seconds) {
+                        int32_t* nstat, const eclcc_params* params, double* seconds) try {
   int rc = check_csr_args(nodes, edges, nindex, nlist);
   if (rc) return rc;
   if (!nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
@@ -395,11 +419,15 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
                       ECLCC_ERR_COPY, "copying from device failed")))
     return rc;
   return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
-                  int32_t* nstat) {
+                  int32_t* nstat) try {
   return eclcc_compute_timed(nodes, edges, nindex, nlist, nstat, nullptr, nullptr);
+} catch (...) {
+  return host_exception();
 }
 
 }  // extern "C"
@@ -470,7 +498,7 @@ extern "C" {
 
 int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edges,
                        const int32_t* const* nindex, const int32_t* const* nlist,
-                       const eclcc_params* params, eclcc_batch** out) {
+                       const eclcc_params* params, eclcc_batch** out) try {
   if (!out) return set_error(ECLCC_ERR_ARG, "null output pointer");
   *out = nullptr;
   if (ngraphs < 1 || !nodes || !edges || !nindex || !nlist)
@@ -546,9 +574,11 @@ int eclcc_batch_create(int32_t ngraphs, const int32_t* nodes, const int32_t* edg
   }
   *out = b.release();
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph) {
+int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph) try {
   if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
   hipStream_t s = (hipStream_t)stream;
   const KParams p = b->params();
@@ -579,9 +609,11 @@ int eclcc_batch_run(eclcc_batch* b, void* stream, int use_graph) {
       return rc;
   }
   return check_hip(hipGraphLaunch(b->exec, s), ECLCC_ERR_LAUNCH, "graph launch failed");
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_batch_run_profiled(eclcc_batch* b, void* stream, float* kernel_ms, float* total_ms) {
+int eclcc_batch_run_profiled(eclcc_batch* b, void* stream, float* kernel_ms, float* total_ms) try {
   if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
   hipEvent_t ev[kNumKernels + 1] = {};
   int rc = ECLCC_OK;
@@ -606,9 +638,11 @@ int eclcc_batch_run_profiled(eclcc_batch* b, void* stream, float* kernel_ms, flo
   for (int i = 0; i <= kNumKernels; i++)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
   return rc;
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
+int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) try {
   if (!b || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
   const GraphDesc& d = b->desc[g];
   eclcc_params prm;
@@ -623,9 +657,11 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) {
   prm.sample_shift = b->tuning.sample_shift;
   return eclcc_run_device(d.n, d.m, d.nidx, d.nlist, d.nstat, b->d_work1, b->work1_bytes, &prm,
                           stream);
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) {
+int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) try {
   if (!b) return set_error(ECLCC_ERR_ARG, "null batch");
   Tuning t;
   int rc = resolve_tuning(params, &t);
@@ -666,14 +702,18 @@ int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) {
     b->graph = nullptr;
   }
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat) {
+int eclcc_batch_labels(eclcc_batch* b, int32_t g, int32_t* nstat) try {
   if (!b || !nstat || g < 0 || g >= b->ngraphs) return set_error(ECLCC_ERR_ARG, "bad batch or graph index");
   const GraphDesc& d = b->desc[g];
   return check_hip(hipMemcpy(nstat, d.nstat, sizeof(int32_t) * (size_t)d.n,
                              hipMemcpyDeviceToHost),
                    ECLCC_ERR_COPY, "copying from device failed");
+} catch (...) {
+  return host_exception();
 }
 
 int32_t* eclcc_batch_device_labels(eclcc_batch* b, int32_t g) {

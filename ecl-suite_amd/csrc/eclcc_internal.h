@@ -126,6 +126,9 @@ int hub_block_residency(int block_threads);
 
 // Status helpers (thread-local message).
 int set_error(int code, const std::string& msg);
+// Inside a catch handler: the status for the exception in flight (host
+// exceptions never cross the C ABI).
+int host_exception();
 int check_hip(hipError_t e, int code, const char* what);
 int resolve_tuning(const eclcc_params* in, Tuning* out);
 int device_cu_count();

@@ -22,6 +22,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -163,6 +164,14 @@ struct eclcc_server {
   // Launches one batch on stream k.  Returns a status; on failure the
   // requests are completed with that status by the caller.
   int launch(Batch* b, int k) {
+    try {
+      return launch_tables(b, k);
+    } catch (...) {
+      return host_exception();  // the batch's tables could not be built
+    }
+  }
+
+  int launch_tables(Batch* b, int k) {
     hipStream_t s = streams[k];
     const int G = (int)b->reqs.size();
     // layout inside the batch's arena block
@@ -301,12 +310,26 @@ struct eclcc_server {
         std::unique_lock<std::mutex> lock(mu);
         if (stop && pending.empty() && !busy) return;
         if (free_k >= 0 && !pending.empty()) {
-          nb = new Batch;
+          // the batch and its request list are allocated before any request
+          // leaves the queue; if the host is out of memory the requests wait
+          nb = new (std::nothrow) Batch;
+          if (nb) {
+            try {
+              nb->reqs.reserve((size_t)cfg.max_batch);
+            } catch (...) {
+              delete nb;
+              nb = nullptr;
+            }
+          }
+        }
+        if (nb) {
           size_t sbytes = 0, req_bytes = 0, exact_d = 0, tab_bytes = 0;
           long long vb = 0, hub_cap = 0;
-          while (!pending.empty() && (int)nb->reqs.size() < cfg.max_batch &&
+          // the requests stay queued until the batch has its memory
+          size_t take = 0;
+          while (take < pending.size() && (int)nb->reqs.size() < cfg.max_batch &&
                  inflight + (int64_t)nb->reqs.size() < cfg.max_inflight) {
-            Request* r = pending.front();
+            Request* r = pending[take];
             const long long vb2 = vb + ((long long)r->n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
             const long long cap2 = hub_cap + hub_capacity(r->m, tuning.wave_max, 1);
             const size_t req2 = req_bytes + request_device_bytes(r->n, r->m, r->resident);
@@ -324,8 +347,8 @@ struct eclcc_server {
             hub_cap = cap2;
             req_bytes = req2;
             exact_d = need_d;
-            nb->reqs.push_back(r);
-            pending.pop_front();
+            nb->reqs.push_back(r);  // within the reserved capacity
+            take++;
           }
           if (nb->reqs.empty()) {
             delete nb;
@@ -335,11 +358,11 @@ struct eclcc_server {
             const bool ok2 = ok && stage.alloc(sbytes + tab_bytes, &nb->stage_off);
             if (!ok2) {
               if (ok) dev.release(nb->dev_off, exact_d);
-              // fragmented: put the requests back and wait for a batch to retire
-              for (auto it = nb->reqs.rbegin(); it != nb->reqs.rend(); ++it) pending.push_front(*it);
+              // fragmented: the requests stay queued until a batch retires
               delete nb;
               nb = nullptr;
             } else {
+              pending.erase(pending.begin(), pending.begin() + (std::ptrdiff_t)take);
               nb->dev_bytes = exact_d;
               nb->stage_bytes = sbytes + tab_bytes;
               nb->table_off = nb->stage_off + sbytes;
@@ -380,7 +403,7 @@ struct eclcc_server {
 
 extern "C" {
 
-int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* params, eclcc_server** out) {
+int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* params, eclcc_server** out) try {
   if (!out) return set_error(ECLCC_ERR_ARG, "null output pointer");
   *out = nullptr;
   std::unique_ptr<eclcc_server> s(new (std::nothrow) eclcc_server);
@@ -417,6 +440,8 @@ int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* para
   s->worker = std::thread([raw] { raw->run(); });
   *out = s.release();
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
 }  // extern "C"
@@ -449,8 +474,18 @@ int64_t submit_request(eclcc_server* s, int32_t nodes, int32_t edges, const int3
       return set_error(ECLCC_ERR_STATE, "server is stopping");
     }
     r->id = s->next_id++;
-    s->reqs[r->id] = r;
-    s->pending.push_back(r);
+    try {  // queued in both or in neither
+      s->reqs[r->id] = r;
+      try {
+        s->pending.push_back(r);
+      } catch (...) {
+        s->reqs.erase(r->id);
+        throw;
+      }
+    } catch (...) {
+      delete r;
+      throw;
+    }
   }
   s->cv_work.notify_one();
   return r->id;
@@ -461,20 +496,24 @@ int64_t submit_request(eclcc_server* s, int32_t nodes, int32_t edges, const int3
 extern "C" {
 
 int64_t eclcc_server_submit(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
-                            const int32_t* nlist, int32_t* labels_out) {
+                            const int32_t* nlist, int32_t* labels_out) try {
   return submit_request(s, nodes, edges, nindex, nlist, labels_out, false);
+} catch (...) {
+  return host_exception();
 }
 
 int64_t eclcc_server_submit_device(eclcc_server* s, int32_t nodes, int32_t edges,
                                    const int32_t* d_nindex, const int32_t* d_nlist,
-                                   int32_t* d_labels) {
+                                   int32_t* d_labels) try {
   return submit_request(s, nodes, edges, d_nindex, d_nlist, d_labels, true);
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_server_submit_device_many(eclcc_server* s, int32_t count, const int32_t* nodes,
                                     const int32_t* edges, const int32_t* const* d_nindex,
                                     const int32_t* const* d_nlist, int32_t* const* d_labels,
-                                    int64_t* ids) {
+                                    int64_t* ids) try {
   if (!s) return set_error(ECLCC_ERR_ARG, "null server");
   if (count < 0 || (count > 0 && (!nodes || !edges || !d_nindex || !d_nlist || !d_labels || !ids)))
     return set_error(ECLCC_ERR_ARG, "bad request arguments");
@@ -512,19 +551,38 @@ int eclcc_server_submit_device_many(eclcc_server* s, int32_t count, const int32_
       for (Request* q : rs) delete q;
       return set_error(ECLCC_ERR_STATE, "server is stopping");
     }
-    for (int32_t i = 0; i < count; i++) {
-      Request* r = rs[(size_t)i];
-      r->id = s->next_id++;
-      ids[i] = r->id;
-      s->reqs[r->id] = r;
-      s->pending.push_back(r);
+    // all queued or none: a host allocation failure part way takes the
+    // ones already queued back out
+    int32_t queued = 0;
+    try {
+      for (; queued < count; queued++) {
+        Request* r = rs[(size_t)queued];
+        r->id = s->next_id++;
+        ids[queued] = r->id;
+        s->reqs[r->id] = r;
+        try {
+          s->pending.push_back(r);
+        } catch (...) {
+          s->reqs.erase(r->id);
+          throw;
+        }
+      }
+    } catch (...) {
+      for (int32_t i = queued - 1; i >= 0; i--) {
+        s->pending.pop_back();
+        s->reqs.erase(rs[(size_t)i]->id);
+      }
+      for (Request* q : rs) delete q;
+      throw;
     }
   }
   s->cv_work.notify_one();
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms) {
+int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* latency_ms) try {
   if (!s) return set_error(ECLCC_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lock(s->mu);
   auto it = s->reqs.find(id);
@@ -542,10 +600,12 @@ int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* l
   delete r;
   if (status != ECLCC_OK) return set_error(status, "request failed");
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int eclcc_server_stats(eclcc_server* s, int64_t* completed, int64_t* batches, size_t* peak_bytes,
-                       int32_t* max_batch_seen) {
+                       int32_t* max_batch_seen) try {
   if (!s) return set_error(ECLCC_ERR_ARG, "null server");
   std::lock_guard<std::mutex> lock(s->mu);
   if (completed) *completed = s->completed;
@@ -553,6 +613,8 @@ int eclcc_server_stats(eclcc_server* s, int64_t* completed, int64_t* batches, si
   if (peak_bytes) *peak_bytes = s->peak_bytes;
   if (max_batch_seen) *max_batch_seen = s->max_batch_seen;
   return ECLCC_OK;
+} catch (...) {
+  return host_exception();
 }
 
 void eclcc_server_free(eclcc_server* s) {

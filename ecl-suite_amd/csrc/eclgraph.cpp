@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <new>
 #include <string>
@@ -42,6 +43,21 @@ thread_local int t_threads = 0;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+// Host exceptions never cross the C ABI: every entry point that can meet
+// one (a host allocation that fails, a thread that cannot start) turns it
+// into a status here.
+int host_exception() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(ECLGRAPH_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ECLGRAPH_ERR_ALLOC, std::string("host resources: ") + e.what());
+  } catch (...) {
+    return fail(ECLGRAPH_ERR_ALLOC, "host error");
+  }
 }
 
 int worker_count() {
@@ -255,7 +271,7 @@ const int32_t* eclgraph_eweight(const eclgraph_csr* g) { return g ? g->eweight.g
 
 void eclgraph_free(eclgraph_csr* g) { delete g; }
 
-int eclgraph_read(const char* path, eclgraph_csr** out) {
+int eclgraph_read(const char* path, eclgraph_csr** out) try {
   if (!path || !out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
   FILE* f = std::fopen(path, "rb");
@@ -290,9 +306,11 @@ int eclgraph_read(const char* path, eclgraph_csr** out) {
     return fail(ECLGRAPH_ERR_FORMAT, "neighbor index list inconsistent with edge count");
   *out = g.release();
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
-int eclgraph_write(const eclgraph_csr* g, const char* path) {
+int eclgraph_write(const eclgraph_csr* g, const char* path) try {
   if (!g || !path) return fail(ECLGRAPH_ERR_ARG, "null argument");
   if (g->nodes < 1 || g->edges < 0) return fail(ECLGRAPH_ERR_ARG, "node or edge count too low");
   if (g->edges > INT32_MAX) return fail(ECLGRAPH_ERR_RANGE, "edge count exceeds int32 format");
@@ -310,10 +328,12 @@ int eclgraph_write(const eclgraph_csr* g, const char* path) {
     return fail(ECLGRAPH_ERR_IO, "failed to write edge weights");
   if (std::fflush(f) != 0) return fail(ECLGRAPH_ERR_IO, "failed to flush output file");
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_mst_weights(int32_t nodes, const int32_t* nindex, const int32_t* nlist,
-                         int32_t* eweight) {
+                         int32_t* eweight) try {
   if (nodes < 1 || !nindex || (nindex[nodes] > 0 && (!nlist || !eweight)))
     return fail(ECLGRAPH_ERR_ARG, "bad argument");
   parallel_for(nodes, 1 << 14, [&](int64_t lo, int64_t hi) {
@@ -328,10 +348,12 @@ int eclgraph_mst_weights(int32_t nodes, const int32_t* nindex, const int32_t* nl
     }
   });
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const int32_t* nlist,
-                   int flags, int64_t* where) {
+                   int flags, int64_t* where) try {
   if (where) *where = -1;
   if (!nindex || (edges > 0 && !nlist)) return fail(ECLGRAPH_ERR_ARG, "null argument");
   if (nodes < 1 || edges < 0) return fail(ECLGRAPH_ERR_ARG, "node or edge count too low");
@@ -384,10 +406,12 @@ int eclgraph_check(int32_t nodes, int64_t edges, const int32_t* nindex, const in
       return bad(first_asym.load(), "entry without its reverse (graph is not symmetric)");
   }
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_from_arrays(int32_t nodes, int64_t edges, const int32_t* nindex,
-                         const int32_t* nlist, const int32_t* eweight, eclgraph_csr** out) {
+                         const int32_t* nlist, const int32_t* eweight, eclgraph_csr** out) try {
   if (!out || !nindex || (edges > 0 && !nlist)) return fail(ECLGRAPH_ERR_ARG, "null argument");
   if (nodes < 1 || edges < 0) return fail(ECLGRAPH_ERR_ARG, "node or edge count too low");
   if (edges > INT32_MAX) return fail(ECLGRAPH_ERR_RANGE, "edge count exceeds int32 format");
@@ -407,17 +431,21 @@ int eclgraph_from_arrays(int32_t nodes, int64_t edges, const int32_t* nindex,
   }
   *out = g.release();
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_from_edges(int32_t nodes, int64_t nedges, const int32_t* src, const int32_t* dst,
-                        int flags, eclgraph_csr** out) {
+                        int flags, eclgraph_csr** out) try {
   if (!out || (nedges > 0 && (!src || !dst))) return fail(ECLGRAPH_ERR_ARG, "null argument");
   if (nodes < 1 || nedges < 0) return fail(ECLGRAPH_ERR_ARG, "node or edge count too low");
   *out = nullptr;
   return build_csr(nodes, nedges, PairArrays{src, dst}, flags, out);
+} catch (...) {
+  return host_exception();
 }
 
-int eclgraph_read_mtx(const char* path, int flags, eclgraph_csr** out) {
+int eclgraph_read_mtx(const char* path, int flags, eclgraph_csr** out) try {
   if (!path || !out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
   FILE* f = std::fopen(path, "r");
@@ -459,9 +487,11 @@ int eclgraph_read_mtx(const char* path, int flags, eclgraph_csr** out) {
     }
   }
   return build_csr((int32_t)n, (int64_t)s.size(), PairArrays{s.data(), d.data()}, flags, out);
+} catch (...) {
+  return host_exception();
 }
 
-int eclgraph_gen_er(int32_t n, int64_t pairs, uint64_t seed, eclgraph_csr** out) {
+int eclgraph_gen_er(int32_t n, int64_t pairs, uint64_t seed, eclgraph_csr** out) try {
   if (!out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
   if (n < 1 || pairs < 0) return fail(ECLGRAPH_ERR_ARG, "n must be >= 1 and pairs >= 0");
@@ -494,10 +524,12 @@ int eclgraph_gen_er(int32_t n, int64_t pairs, uint64_t seed, eclgraph_csr** out)
     have++;
   }
   return build_csr(n, pairs, PairArrays{su.data(), sv.data()}, ECLGRAPH_SYMMETRIZE, out);
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_gen_rmat(int scale, int32_t n, int64_t samples, double a, double b, double c,
-                      uint64_t seed, int permute, eclgraph_csr** out) {
+                      uint64_t seed, int permute, eclgraph_csr** out) try {
   if (!out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
   if (scale < 1 || scale > 30) return fail(ECLGRAPH_ERR_ARG, "scale must be in [1, 30]");
@@ -557,10 +589,12 @@ int eclgraph_gen_rmat(int scale, int32_t n, int64_t samples, double a, double b,
   }
   return build_csr(n, samples, PairArrays{src.data(), dst.data()},
                    ECLGRAPH_SYMMETRIZE | ECLGRAPH_DROP_SELF | ECLGRAPH_DEDUPE, out);
+} catch (...) {
+  return host_exception();
 }
 
 int eclgraph_gen_grid3d(int32_t nx, int32_t ny, int32_t nz, double keep_p, uint64_t seed,
-                        int permute, eclgraph_csr** out) {
+                        int permute, eclgraph_csr** out) try {
   if (!out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
   if (nx < 1 || ny < 1 || nz < 1) return fail(ECLGRAPH_ERR_ARG, "grid dimensions must be >= 1");
@@ -627,6 +661,8 @@ int eclgraph_gen_grid3d(int32_t nx, int32_t ny, int32_t nz, double keep_p, uint6
   });
   *out = g.release();
   return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
 }
 
 }  // extern "C"

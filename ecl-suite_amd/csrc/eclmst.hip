@@ -39,7 +39,9 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <exception>
 #include <mutex>
+#include <new>
 #include <string>
 
 #include "../../include/eclmst.h"
@@ -497,6 +499,20 @@ int hip_fail(hipError_t e, int code, const char* what) {
   return fail(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Host exceptions never cross the C ABI: every entry point that can meet
+// one (a host allocation that fails) turns it into a status here.
+int host_exception() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(ECLMST_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ECLMST_ERR_ALLOC, std::string("host resources: ") + e.what());
+  } catch (...) {
+    return fail(ECLMST_ERR_ALLOC, "host error");
+  }
+}
+
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 unsigned list_cap(long long m) { return (unsigned)std::min<long long>(m / 2 + 64, 0xFFFFFF00ll); }
@@ -724,16 +740,20 @@ int32_t eclmst_threshold(int32_t nodes, int32_t edges, const int32_t* eweight) {
   return s[idx];
 }
 
-size_t eclmst_workspace_bytes(int32_t nodes, int32_t edges) {
+size_t eclmst_workspace_bytes(int32_t nodes, int32_t edges) try {
   if (nodes < 1 || edges < 0) return 0;
   return layout_bytes(nodes, edges);
+} catch (...) {
+  return (host_exception(), size_t(0));
 }
 
 int eclmst_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex, const int32_t* d_nlist,
                       const int32_t* d_eweight, uint8_t* d_in_mst, void* d_work, size_t work_bytes,
-                      int32_t threshold, void* stream, float* device_ms) {
+                      int32_t threshold, void* stream, float* device_ms) try {
   return run_device(nodes, edges, d_nindex, d_nlist, d_eweight, d_in_mst, d_work, work_bytes,
                     threshold, static_cast<hipStream_t>(stream), device_ms);
+} catch (...) {
+  return host_exception();
 }
 
 static int compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
@@ -786,14 +806,18 @@ static int compute(int32_t nodes, int32_t edges, const int32_t* nindex, const in
 }
 
 int eclmst_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
-                   const int32_t* eweight, uint8_t* in_mst) {
+                   const int32_t* eweight, uint8_t* in_mst) try {
   return compute(nodes, edges, nindex, nlist, eweight, in_mst, nullptr);
+} catch (...) {
+  return host_exception();
 }
 
 int eclmst_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex,
                          const int32_t* nlist, const int32_t* eweight, uint8_t* in_mst,
-                         double* seconds) {
+                         double* seconds) try {
   return compute(nodes, edges, nindex, nlist, eweight, in_mst, seconds);
+} catch (...) {
+  return host_exception();
 }
 
 }  // extern "C"

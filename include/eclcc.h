@@ -21,7 +21,8 @@
  *
  * Conventions: every function returns ECLCC_OK (0) or a negative status and
  * never exits the process; eclcc_last_error() gives the message (per host
- * thread).  `stream` arguments are hipStream_t passed as void* (NULL = the
+ * thread).  No C++ exception leaves the library: a host allocation that
+ * fails is ECLCC_ERR_ALLOC.  `stream` arguments are hipStream_t passed as void* (NULL = the
  * null stream).  All device pointers belong to the current HIP device.
  */
 #ifndef ECLCC_ABI_H

@@ -14,6 +14,8 @@
  * eclgraph_last_error() returns a message (thread-local).  Nothing here calls
  * exit(): the reference's reader exits the process on any error
  * (ECLgraph.h:53-72); this library reports the same conditions as statuses.
+ * No C++ exception leaves the library: a host allocation that fails, or
+ * worker threads that cannot start, is ECLGRAPH_ERR_ALLOC.
  */
 #ifndef ECLGRAPH_ABI_H
 #define ECLGRAPH_ABI_H

@@ -22,7 +22,8 @@
  *   by (weight, j, ...) (:122-131).
  *
  * Conventions as in eclcc.h: 0 or a negative ECLMST_ERR_* status, never an
- * exit; eclmst_last_error() gives the message (per host thread).  Calls
+ * exit or an escaping C++ exception (a failed host allocation is
+ * ECLMST_ERR_ALLOC); eclmst_last_error() gives the message (per host thread).  Calls
  * from several host threads are safe and run one at a time (they share the
  * small pinned buffer through which each round's list size is read).
  */

@@ -247,3 +247,29 @@ def test_sampled_vertex_count_matches_the_rule():
             want += int(lst[0] != v)
     assert roofline.bins(g.nindex, g.nlist, 16, 2048)["sampled"][0] == want
     assert roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048)["sample"] == 4 * want
+
+
+def test_host_allocation_failure_is_a_status_not_an_abort():
+    # under a 1.5 GiB address-space limit the ER generator's 2 GiB pair
+    # table cannot be allocated: the library returns ECLGRAPH_ERR_ALLOC
+    # (no C++ exception crosses the C ABI, so the process does not abort)
+    import sys
+
+    from eclcc import _native as N
+
+    code = r"""
+import ctypes, resource, sys
+resource.setrlimit(resource.RLIMIT_AS, (1536 << 20, 1536 << 20))
+L = ctypes.CDLL(sys.argv[1])
+L.eclgraph_gen_er.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
+                              ctypes.POINTER(ctypes.c_void_p)]
+L.eclgraph_last_error.restype = ctypes.c_char_p
+out = ctypes.c_void_p()
+rc = L.eclgraph_gen_er(1 << 20, 100_000_000, 1, ctypes.byref(out))
+print(rc, bool(out.value), L.eclgraph_last_error().decode())
+"""
+    r = subprocess.run([sys.executable, "-c", code, str(N.LIB_DIR / "libeclgraph.so")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rc, made, msg = r.stdout.strip().split(" ", 2)
+    assert int(rc) == -3 and made == "False" and "allocation" in msg, r.stdout
