@@ -174,7 +174,15 @@ def plain(args) -> dict:
             kt = t.get("kernels", {}).get(dom)
             if kt is not None:
                 out["roofline"]["traffic"] = kt["fetch_bytes"] + kt["write_bytes"]
+                # the bytes the kernel moves (line granularity included) over the
+                # same launch time: how close its gathers run to the HBM peak
+                gbs = out["roofline"]["traffic"] / (kernel_ms[dom] * 1e-3) / 1e9
+                out["roofline"]["traffic_GBps"] = gbs
+                out["roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
             out["pass_roofline"]["traffic"] = t["bytes_per_pass"]
+            gbs = t["bytes_per_pass"] / (pass_ms * 1e-3) / 1e9
+            out["pass_roofline"]["traffic_GBps"] = gbs
+            out["pass_roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
             out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     # the same graph with every adjacency list read (giant_filter 1)
     if dg.params.giant_filter == 2 and not args.no_compare:
