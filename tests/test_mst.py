@@ -283,3 +283,46 @@ def test_mst_shared_accesses_go_through_atomic_helpers():
               "__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)",
               "__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)"):
         assert f in prod, f
+
+
+def _forest_of_multigraph(g):
+    """The forest of a multigraph whose simple graph has no cycle and whose
+    lists are sorted where they hold an entry to a larger neighbour (the
+    builder in graphs.max_entries_multigraph): every distinct edge i < k is
+    in it, at the first copy in i's list (all copies weigh the same, and
+    ties go to the smaller entry index).  Returns the entry indices."""
+    pos = []
+    for i in range(g.nodes):
+        b, e = int(g.nindex[i]), int(g.nindex[i + 1])
+        seg = g.nlist[b:e]
+        if e == b or int(seg[-1]) <= i:
+            continue
+        for k in np.unique(seg[[0, np.searchsorted(seg, i + 1), e - b - 1]]):
+            if k > i:
+                pos.append(b + int(np.searchsorted(seg, k)))
+    return np.asarray(sorted(set(pos)), dtype=np.int64)
+
+
+def test_forest_of_multigraph_helper_equals_the_oracle():
+    import graphs as G
+
+    g, _ = G.max_entries_multigraph(chain=12, split=5, tail_loops=11, x=7, edges=10001)
+    w = mst.mst_weights(g)
+    ref = oracle.mst(g.nindex, g.nlist, w)
+    assert np.array_equal(np.flatnonzero(ref), _forest_of_multigraph(g))
+
+
+@pytest.mark.gpu
+def test_gpu_mst_at_the_largest_int32_edge_count(cuda):
+    # 2^31 - 1 entries (8 GiB of nlist and 8 GiB of weights) with the
+    # reference's default weights and threshold: the forest is every edge
+    # of the two paths and the extra vertex's edge, each at its first copy
+    import graphs as G
+
+    g, _ = G.max_entries_multigraph()
+    w = mst.mst_weights(g)
+    want = _forest_of_multigraph(g)
+    announce(f"eclmst_compute {g.name} n={g.nodes} threshold {mst.threshold(g, w)}")
+    got = mst.compute_mst(g, w)
+    assert int(got.sum(dtype=np.int64)) == want.size == g.nodes - 3  # three components
+    assert bool(got[want].all())
