@@ -128,3 +128,50 @@ def max_entries_multigraph(chain=40, split=20, tail_loops=101, x=30, edges=INT32
     expected[z] = split
     expected[z + 1] = z + 1
     return g, expected
+
+
+def random_structure(seed, n_hi=3000):
+    """A seeded small graph of an irregular shape, in the CSR form the
+    reference accepts but its generators never write: lists in random order,
+    duplicate entries and self-loops.  The shape is one of random pairs,
+    stars (a hub at a random ID, some leaves cross-linked), a permuted path,
+    or shuffled cliques; isolated vertices occur in all of them."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, n_hi))
+    kind = seed % 4
+    if kind == 0:
+        k = int(rng.integers(0, 4 * n + 1))
+        a, b = rng.integers(0, n, k), rng.integers(0, n, k)
+    elif kind == 1:
+        hub = int(rng.integers(0, n))
+        leaves = rng.choice(n, size=int(rng.integers(0, n)), replace=False)
+        extra = int(rng.integers(0, n // 4 + 1))
+        a = np.concatenate([np.full(leaves.size, hub), rng.choice(n, extra)])
+        b = np.concatenate([leaves, rng.choice(n, extra)])
+    elif kind == 2:
+        perm = rng.permutation(n)
+        cut = rng.random(max(n - 1, 0)) < 0.9  # keep most path edges
+        a, b = perm[:-1][cut], perm[1:][cut]
+    else:
+        size = int(rng.integers(2, 9))
+        perm = rng.permutation(n)
+        pa, pb = [], []
+        for c0 in range(0, n - n % size, size):
+            grp = perm[c0:c0 + size]
+            i, j = np.triu_indices(size, 1)
+            pa.append(grp[i])
+            pb.append(grp[j])
+        a = np.concatenate(pa) if pa else np.zeros(0, np.int64)
+        b = np.concatenate(pb) if pb else np.zeros(0, np.int64)
+    dup = rng.random(a.size) < 0.1  # some edges twice
+    a = np.concatenate([a, a[dup]])
+    b = np.concatenate([b, b[dup]])
+    src = np.concatenate([a, b]).astype(np.int64)
+    dst = np.concatenate([b, a]).astype(np.int64)
+    order = rng.permutation(src.size)  # random order within each list
+    src, dst = src[order], dst[order]
+    by = np.argsort(src, kind="stable")
+    nlist = dst[by].astype(np.int32)
+    nindex = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=nindex[1:])
+    return Graph(nindex.astype(np.int32), nlist, name=f"random-structure-{seed}-k{kind}-n{n}")

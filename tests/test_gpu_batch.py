@@ -159,3 +159,30 @@ def test_server_bulk_device_requests(cuda):
         with pytest.raises(Exception):
             s.submit_device_many(bad)
         assert s.stats()["completed"] == done
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=64),
+                                dict(giant_filter=1, sample_shift=3),
+                                dict(giant_filter=2, thread_max_deg=0, wave_max_deg=1, path_mode=0)],
+                         ids=["default", "f0-small-bins", "f1-shift3", "f2-all-hubs"])
+def test_batch_of_irregular_graphs(cuda, kw):
+    # unsorted lists, duplicates, self-loops, stars with the hub anywhere,
+    # permuted paths, shuffled cliques, sizes from 1 to 3000 (most not a
+    # multiple of 64): each graph's labels equal the oracle's, in the batch
+    # and in the one-graph-per-launch form
+    import graphs as G
+    import torch
+
+    gs = [G.random_structure(s) for s in range(48)]
+    expect = [oracle.cc_labels(g.nindex, g.nlist) for g in gs]
+    announce(f"batch of {len(gs)} irregular graphs, params {kw}")
+    with Batch(gs, default_params(**kw)) as b:
+        b.run(use_graph=True)
+        torch.cuda.synchronize()
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), g.name
+        for i in range(len(gs)):
+            b.run_single(i)
+        torch.cuda.synchronize()
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), ("single", g.name)

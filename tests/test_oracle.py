@@ -144,3 +144,22 @@ def test_max_entries_multigraph_builder_small():
     assert np.array_equal(oracle.cc_labels(g.nindex, g.nlist), exp)
     z = 12
     assert list(g.nlist[g.nindex[z]:g.nindex[z + 1]]) == [z] * 11 + [7]
+
+
+def test_random_structure_builder_is_symmetric_and_oracle_equals_scipy():
+    # the irregular graphs of the batched GPU test: symmetric CSR (lists
+    # unsorted, with duplicates and self-loops), and the oracle's labels
+    # equal scipy's components canonicalised to minimum IDs
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    for seed in range(16):
+        g = G.random_structure(seed)
+        n = g.nodes
+        src = np.repeat(np.arange(n, dtype=np.int64), np.diff(g.nindex))
+        assert np.array_equal(np.sort(src * n + g.nlist), np.sort(g.nlist.astype(np.int64) * n + src))
+        a = sp.csr_matrix((np.ones(g.edges), (src, g.nlist)), shape=(n, n))
+        c, lab = cg.connected_components(a, directed=False)
+        mins = np.full(c, n)
+        np.minimum.at(mins, lab, np.arange(n))
+        assert np.array_equal(oracle.cc_labels(g.nindex, g.nlist), mins[lab]), g.name
