@@ -326,3 +326,29 @@ def test_gpu_mst_at_the_largest_int32_edge_count(cuda):
     got = mst.compute_mst(g, w)
     assert int(got.sum(dtype=np.int64)) == want.size == g.nodes - 3  # three components
     assert bool(got[want].all())
+
+
+@pytest.mark.gpu
+def test_gpu_mst_on_irregular_graphs(cuda):
+    # unsorted lists, duplicate entries, self-loops, stars, paths, cliques;
+    # the default weights and random weights from a small range (many ties,
+    # broken by the entry index): the forest equals the oracle's entry for
+    # entry
+    import graphs as G
+
+    rng = np.random.default_rng(11)
+    for seed in range(24):
+        g = G.random_structure(seed)
+        for name, w in (("default", mst.mst_weights(g)), ("ties", None)):
+            # a duplicate entry must weigh the same in both directions
+            src = np.repeat(np.arange(g.nodes), np.diff(g.nindex))
+            lo = np.minimum(src, g.nlist).astype(np.int64)
+            hi = np.maximum(src, g.nlist).astype(np.int64)
+            if name == "ties":
+                key = lo * g.nodes + hi
+                _, inv = np.unique(key, return_inverse=True)
+                w = rng.integers(0, 4, inv.max() + 1 if inv.size else 1).astype(np.int32)[inv]
+            announce(f"eclmst_compute {g.name} weights {name}")
+            want = oracle.mst(g.nindex, g.nlist, w)
+            got = mst.compute_mst(g, w)
+            assert np.array_equal(got[:g.edges], want[:g.edges]), (g.name, name)
