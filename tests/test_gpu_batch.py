@@ -186,3 +186,20 @@ def test_batch_of_irregular_graphs(cuda, kw):
         torch.cuda.synchronize()
         for g, e, lab in zip(gs, expect, b.all_labels()):
             assert np.array_equal(lab, e), ("single", g.name)
+
+
+def test_server_on_irregular_graphs_under_a_tight_budget(cuda):
+    # requests of every irregular shape (down to one vertex) submitted at
+    # once to a server whose small HBM budget forces many small batches:
+    # each result equals the oracle's
+    import graphs as G
+
+    gs = [G.random_structure(s) for s in range(100, 164)]
+    announce(f"server with {len(gs)} irregular requests, 8 MiB budget")
+    with Server(hbm_budget_bytes=8 << 20, streams=4, max_batch=8) as s:
+        ids = [s.submit(g) for g in gs]
+        for g, rid in zip(gs, ids):
+            lab, lat = s.wait(rid, timeout_ms=60000)
+            assert np.array_equal(lab, oracle.cc_labels(g.nindex, g.nlist)), g.name
+        st = s.stats()
+        assert st["completed"] == len(gs) and st["batches"] >= len(gs) // 8
