@@ -286,3 +286,15 @@ def test_lists_ending_at_the_largest_int32_edge_count(cuda):
     finally:
         del dg
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(giant_filter=0, thread_max_deg=1, wave_max_deg=8, hub_chunk=64),
+                                dict(giant_filter=1, path_mode=0), dict(sample_shift=2)],
+                         ids=["default", "f0-small-bins", "f1-halving", "shift2"])
+def test_plain_on_irregular_graphs(cuda, kw):
+    # unsorted lists, duplicates, self-loops, hubs at any ID, permuted paths,
+    # shuffled cliques, 1 to 3000 vertices
+    announce(f"plain eclcc_compute on irregular graphs, params {kw}")
+    for seed in range(200, 232):
+        g = G.random_structure(seed)
+        check(g, compute(g, default_params(**kw)))
