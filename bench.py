@@ -340,7 +340,7 @@ def mst_bench(args) -> dict:
         times.append(time.perf_counter() - t1)
     got_s = mst.compute_mst(gs, ws)
     B = 4 * (g.nodes + 1) + 8 * g.edges + g.edges
-    return {
+    out = {
         "metric": "MST edges/s on RMAT-24 (next path after CC)",
         "mode": "mst",
         "value": g.edges / (ms * 1e-3),
@@ -384,6 +384,16 @@ def mst_bench(args) -> dict:
             "seconds": statistics.median(times),
         },
     }
+    tr = REPO / "profiles" / f"mst_{args.workload}_traffic.json"
+    if tr.exists():
+        t = json.loads(tr.read_text())
+        if t.get("nodes") == g.nodes and t.get("edges") == g.edges and t.get("threshold") == thr:
+            gbs = t["bytes_per_forest"] / (ms * 1e-3) / 1e9
+            out["pass_roofline"]["traffic"] = t["bytes_per_forest"]
+            out["pass_roofline"]["traffic_GBps"] = gbs
+            out["pass_roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
+            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+    return out
 
 
 def copy_probe(args) -> dict:
