@@ -673,26 +673,56 @@ int eclcc_batch_set_params(eclcc_batch* b, const eclcc_params* params) try {
     max_m = std::max<long long>(max_m, d.m);
   }
   const size_t wb = workspace_bytes(b->nslots, hub_cap, b->ngraphs);
-  if (wb != b->work_bytes) {
-    if ((rc = check_hip(hipDeviceSynchronize(), ECLCC_ERR_LAUNCH, "kernel execution failed"))) return rc;
-    (void)hipFree(b->d_work);
-    b->device_bytes -= b->work_bytes;
-    b->d_work = nullptr;
-    b->work_bytes = wb;
-    if ((rc = batch_alloc(b, &b->d_work, wb, "batch workspace"))) return rc;
-  }
-  if (!carve_workspace(b->d_work, b->work_bytes, b->nslots, hub_cap, b->ngraphs, &b->w))
-    return set_error(ECLCC_ERR_ALLOC, "workspace carve failed");
   const size_t w1 = workspace_bytes(max_n, hub_capacity(max_m, t.wave_max, 1), 1);
-  if (w1 > b->work1_bytes) {
-    if ((rc = check_hip(hipDeviceSynchronize(), ECLCC_ERR_LAUNCH, "kernel execution failed"))) return rc;
-    (void)hipFree(b->d_work1);
-    b->device_bytes -= b->work1_bytes;
-    b->d_work1 = nullptr;
-    b->work1_bytes = w1;
-    if ((rc = batch_alloc(b, &b->d_work1, w1, "single-graph workspace"))) return rc;
+  // The new buffers are allocated before anything of the batch changes: on
+  // failure the batch keeps its old workspace, tuning and captured graph,
+  // all consistent with each other.
+  void* nw = nullptr;
+  void* nw1 = nullptr;
+  auto drop_new = [&]() {
+    if (nw) (void)hipFree(nw);
+    if (nw1) (void)hipFree(nw1);
+  };
+  auto alloc_new = [&](void** out, size_t bytes, const char* what) -> int {
+    const hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 4));
+    if (e == hipSuccess) return ECLCC_OK;
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_error(ECLCC_ERR_ALLOC, std::string("could not allocate ") + what + ": " +
+                                          hipGetErrorString(e));
+  };
+  if (wb != b->work_bytes && (rc = alloc_new(&nw, wb, "batch workspace"))) return rc;
+  if (w1 > b->work1_bytes && (rc = alloc_new(&nw1, w1, "single-graph workspace"))) {
+    drop_new();
+    return rc;
   }
+  Workspace w;
+  if (!carve_workspace(nw ? nw : b->d_work, nw ? wb : b->work_bytes, b->nslots, hub_cap,
+                       b->ngraphs, &w)) {
+    drop_new();
+    return set_error(ECLCC_ERR_ALLOC, "workspace carve failed");
+  }
+  if ((nw || nw1) &&
+      (rc = check_hip(hipDeviceSynchronize(), ECLCC_ERR_LAUNCH, "kernel execution failed"))) {
+    drop_new();
+    return rc;
+  }
+  // from here on nothing fails: swap the buffers in
+  if (nw) {
+    (void)hipFree(b->d_work);
+    b->device_bytes = b->device_bytes - b->work_bytes + wb;
+    b->d_work = nw;
+    b->work_bytes = wb;
+  }
+  if (nw1) {
+    (void)hipFree(b->d_work1);
+    b->device_bytes = b->device_bytes - b->work1_bytes + w1;
+    b->d_work1 = nw1;
+    b->work1_bytes = w1;
+  }
+  b->w = w;
   b->tuning = t;
+  // the captured graph holds the old workspace pointers and tuning
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;

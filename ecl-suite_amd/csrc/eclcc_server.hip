@@ -50,6 +50,7 @@ struct Request {
   Clock::time_point t_submit;
   double latency_ms = 0;
   int status = ECLCC_OK;
+  std::string error;       // the failure's message when status != ECLCC_OK
   bool done = false;
   // placement inside its batch
   long long slot0 = 0;
@@ -63,6 +64,15 @@ size_t request_device_bytes(int32_t n, int32_t m, bool resident) {
   const size_t tables = align256(sizeof(GraphDesc)) + align256(4 * (slots / kSlotAlign) + 4);
   if (resident) return tables;
   return align256(4 * ((size_t)n + 1)) + align256(4 * (size_t)m) + align256(4 * (size_t)n) + tables;
+}
+
+// Pinned staging bytes of a batch that holds this one request: its labels
+// (host requests only), then the batch's graph table and chunk map, which
+// go up from pinned memory.  The same sums as the worker's admission loop.
+size_t request_stage_bytes(int32_t n, bool resident) {
+  const size_t slots = ((size_t)n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+  const size_t labels = resident ? 0 : align256(4 * (size_t)n);
+  return labels + align256(sizeof(GraphDesc)) + align256(4 * (slots / kSlotAlign));
 }
 
 struct Batch {
@@ -213,13 +223,23 @@ struct eclcc_server {
     return rc;
   }
 
+  // `status` != ECLCC_OK: the worker thread's message (set_error on this
+  // thread) is kept with each request, so the client's wait reports it.
   void finish(Batch* b, int status) {
     const auto now = Clock::now();
+    const char* msg = status == ECLCC_OK ? "" : eclcc_last_error();
     std::lock_guard<std::mutex> lock(mu);
     for (Request* r : b->reqs) {
       if (status == ECLCC_OK && !r->resident)
         std::memcpy(r->out, h_stage + r->stage_off, 4 * (size_t)r->n);
       r->status = status;
+      if (status != ECLCC_OK) {
+        try {
+          r->error = msg;
+        } catch (...) {
+          // out of host memory: the client gets the status without the text
+        }
+      }
       r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
       r->done = true;
       completed++;
@@ -397,15 +417,28 @@ int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* para
 
 namespace {
 
+// Device bytes of a batch that holds this one request (the admission loop's
+// need_d for a first request).
+size_t request_batch_bytes(const eclcc_server* s, int32_t nodes, int32_t edges, bool resident) {
+  return 4096 + request_device_bytes(nodes, edges, resident) +
+         workspace_bytes(((long long)nodes + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
+                         hub_capacity(edges, s->tuning.wave_max, 1), 1);
+}
+
+bool admissible_alone(const eclcc_server* s, int32_t nodes, int32_t edges, bool resident) {
+  return request_batch_bytes(s, nodes, edges, resident) <= s->dev.capacity() &&
+         request_stage_bytes(nodes, resident) <= s->stage.capacity();
+}
+
 int64_t submit_request(eclcc_server* s, int32_t nodes, int32_t edges, const int32_t* nindex,
                        const int32_t* nlist, int32_t* labels_out, bool resident) {
   if (!s) return set_error(ECLCC_ERR_ARG, "null server");
   if (nodes < 1 || edges < 0 || !nindex || (edges > 0 && !nlist) || !labels_out)
     return set_error(ECLCC_ERR_ARG, "bad request arguments");
-  const size_t need = 4096 + request_device_bytes(nodes, edges, resident) +
-                      workspace_bytes(((long long)nodes + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
-                                      hub_capacity(edges, s->tuning.wave_max, 1), 1);
-  if (need > s->dev.capacity() || (!resident && align256(4 * (size_t)nodes) > s->stage.capacity()))
+  // admissible alone, by the worker's own sums (device arena and pinned
+  // staging); a request that is not would stay at the head of the queue
+  // for good and hold up every request behind it
+  if (!admissible_alone(s, nodes, edges, resident))
     return set_error(ECLCC_ERR_BUDGET, "request larger than the HBM budget");
   Request* r = new (std::nothrow) Request;
   if (!r) return set_error(ECLCC_ERR_ALLOC, "host allocation failed");
@@ -470,10 +503,7 @@ int eclcc_server_submit_device_many(eclcc_server* s, int32_t count, const int32_
   for (int32_t i = 0; i < count; i++) {
     if (nodes[i] < 1 || edges[i] < 0 || !d_nindex[i] || (edges[i] > 0 && !d_nlist[i]) || !d_labels[i])
       return set_error(ECLCC_ERR_ARG, "bad request arguments (request " + std::to_string(i) + ")");
-    const size_t need = 4096 + request_device_bytes(nodes[i], edges[i], true) +
-                        workspace_bytes(((long long)nodes[i] + kSlotAlign - 1) / kSlotAlign * kSlotAlign,
-                                        hub_capacity(edges[i], s->tuning.wave_max, 1), 1);
-    if (need > s->dev.capacity())
+    if (!admissible_alone(s, nodes[i], edges[i], true))
       return set_error(ECLCC_ERR_BUDGET,
                        "request " + std::to_string(i) + " larger than the HBM budget");
   }
@@ -545,9 +575,12 @@ int eclcc_server_wait(eclcc_server* s, int64_t id, int32_t timeout_ms, double* l
   }
   if (latency_ms) *latency_ms = r->latency_ms;
   const int status = r->status;
+  std::string msg;
+  if (status != ECLCC_OK) msg.swap(r->error);
   s->reqs.erase(it);
   delete r;
-  if (status != ECLCC_OK) return set_error(status, "request failed");
+  if (status != ECLCC_OK)
+    return set_error(status, msg.empty() ? std::string("request failed") : "request failed: " + msg);
   return ECLCC_OK;
 } catch (...) {
   return host_exception();

@@ -78,6 +78,10 @@ int main(int argc, char* argv[]) {
   const int32_t m = (int32_t)eclgraph_edges(g);
   const int32_t* nidx = eclgraph_nindex(g);
   const int32_t* nlist = eclgraph_nlist(g);
+  // the kernels index nlist and the per-vertex arrays with nindex and the
+  // entries: a decreasing nindex or an entry outside [0, n) stops here, not
+  // on the GPU (the reference does not check them)
+  if (eclgraph_check(n, m, nidx, nlist, 0, nullptr) != 0) die(eclgraph_last_error());
   const int32_t* fw = eclgraph_eweight(g);
   std::vector<int32_t> dw;
   if (!fw) {  // assign weights if needed

@@ -97,6 +97,30 @@ def test_server_rejects_request_over_budget(cuda):
         assert ei.value.status == -6
 
 
+def test_server_rejects_at_submit_what_staging_cannot_hold(cuda):
+    # 8 MiB budget -> 1 MiB of pinned staging.  A batch of one host request
+    # stages its labels (align256(4n)) plus the graph table (256 B) and the
+    # chunk map (4 B per 256 slots).  n = 262,080: the labels alone fit
+    # (1,048,320 B) but not with the tables, so the worker could never admit
+    # it; it must be refused at submit, not left at the head of the queue.
+    # n = 261,000 fills the staging exactly (1,044,224 + 256 + 4,096 B) and
+    # is served, as is a request queued behind the refused one.
+    import graphs as G
+    from eclcc import EclccError
+
+    announce("server staging boundary at an 8 MiB budget")
+    big, fits, small = G.path(262080), G.path(261000), G.path(1000)
+    with Server(hbm_budget_bytes=8 << 20, streams=1) as s:
+        with pytest.raises(EclccError) as ei:
+            s.submit(big)
+        assert ei.value.status == -6
+        r1, r2 = s.submit(fits), s.submit(small)
+        lab, _ = s.wait(r1, timeout_ms=60000)
+        assert np.array_equal(lab, np.zeros(fits.nodes, dtype=np.int32))
+        lab, _ = s.wait(r2, timeout_ms=60000)
+        assert np.array_equal(lab, np.zeros(small.nodes, dtype=np.int32))
+
+
 def test_server_device_resident_requests(cuda):
     import torch
 

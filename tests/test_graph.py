@@ -205,6 +205,38 @@ def test_cli_check_flag(tmp_path):
     assert r.returncode == 255 and "not symmetric" in r.stderr
 
 
+def _raw_egr(path, nindex, nlist):
+    """An .egr file with exactly these arrays (no checks on the way)."""
+    ni = np.asarray(nindex, dtype=np.int32)
+    nl = np.asarray(nlist, dtype=np.int32)
+    with open(path, "wb") as f:
+        np.array([ni.size - 1, nl.size], dtype=np.int32).tofile(f)
+        ni.tofile(f)
+        nl.tofile(f)
+
+
+@pytest.mark.parametrize("exe_name", ["eclcc", "eclmst"])
+def test_clis_reject_broken_index_and_ids_before_the_gpu(tmp_path, exe_name):
+    # a decreasing nindex and an entry outside [0, n) would index the label
+    # array and the lists out of bounds on the GPU; both CLIs stop on the
+    # host with the reference's ERROR line and exit status -1, without --check
+    from eclcc import _native as N
+
+    exe = N.BIN_DIR / exe_name
+    cases = {
+        "decreasing": ([0, 2, 1, 3, 4], [1, 2, 0, 3], "nindex decreases"),
+        "out_of_range": ([0, 1, 2, 3, 4], [1, 0, 7, 2], "out of range"),
+        "negative": ([0, 1, 2, 3, 4], [1, 0, -3, 2], "out of range"),
+    }
+    for name, (ni, nl, msg) in cases.items():
+        path = tmp_path / f"{name}.egr"
+        _raw_egr(path, ni, nl)
+        r = subprocess.run([str(exe), str(path)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 255, (name, r.stdout, r.stderr)
+        assert "ERROR:" in r.stderr and msg in r.stderr, (name, r.stderr)
+        assert "compute time" not in r.stdout and "Dev" not in r.stdout
+
+
 def test_kernel_bytes_partition_the_active_entries():
     from eclcc import roofline
 
