@@ -51,12 +51,13 @@ int check_hip(hipError_t e, int code, const char* what) {
 }
 
 int device_cu_count() {
+  // a fixed table, so that the entry points that must not allocate can call it
   static std::mutex mu;
-  static std::vector<int> cache;
+  static int cache[kMaxDevices] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (dev < 0 || dev >= kMaxDevices) return 0;
   std::lock_guard<std::mutex> lock(mu);
-  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
   if (cache[dev] == 0) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;

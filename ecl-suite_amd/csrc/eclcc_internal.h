@@ -23,6 +23,8 @@ constexpr int kScanBlock = 1024;
 // Vertices sampled per graph to find its largest component.
 constexpr int kPickSamples = 256;
 constexpr int kNumKernels = ECLCC_NUM_KERNELS;
+// HIP devices the per-device tables of the library cover.
+constexpr int kMaxDevices = 64;
 
 // One graph of a batch.  Its arrays may sit anywhere in device memory (packed
 // into one allocation by eclcc_batch, placed in the server's arena, or owned

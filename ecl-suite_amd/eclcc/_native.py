@@ -119,6 +119,7 @@ def _declare_cc(L: C.CDLL) -> C.CDLL:
     L.eclcc_workspace_bytes.argtypes = [i32, i64, PP]
     L.eclcc_workspace_bytes.restype = C.c_size_t
     L.eclcc_run_device.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp]
+    L.eclcc_run_device_noalloc.argtypes = [i32, i32, vp, vp, vp, PP, vp]
     L.eclcc_run_device_profiled.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, PP, vp,
                                             C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.eclcc_run_init_only.argtypes = [i32, i32, vp, vp, vp, vp, C.c_size_t, vp]

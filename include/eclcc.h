@@ -110,6 +110,22 @@ int eclcc_run_device(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                      const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
                      const eclcc_params* params, void* stream);
 
+/* No counterpart in the reference (it allocates on every call,
+   ECL-CC-atomic.cu:290-293).  For callers that cannot allocate device
+   memory: code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc.  Same as eclcc_run_device, but the workspace is a fixed
+   device array of ECLCC_NOALLOC_SCRATCH_BYTES inside the library (one per
+   GPU, held until the process ends); the call allocates nothing.  Graphs
+   whose workspace (eclcc_workspace_bytes) exceeds it are refused with
+   ECLCC_ERR_BUDGET (with the default parameters: about 4.9 bytes per vertex
+   and 0.012 per entry, so about 13 million vertices).  Calls that overlap
+   on one GPU share the array and race: keep them on one stream, or wait
+   for one before starting the next. */
+#define ECLCC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int eclcc_run_device_noalloc(int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                             const int32_t* d_nlist, int32_t* d_nstat,
+                             const eclcc_params* params, void* stream);
+
 /* Kernels of one run, in launch order: init, sample, pick, mark, hubscan,
    compute1, compute2, compute3, flatten. */
 #define ECLCC_NUM_KERNELS 9
