@@ -3,11 +3,14 @@
 
 Default (``python bench.py``): plain ECL-CC on R-MAT scale 24 (edge factor
 16, a,b,c = .57,.19,.19, symmetrised, deduplicated, IDs permuted, seed 24),
-CSR resident in HBM.  One step = one full CC pass (init, compute1, compute2,
-compute3, flatten).  Prints ONE JSON line with edges/s (edges = CSR entries,
-the reference's definition, ECL-CC-atomic.cu:316), the HBM roofline of the
-pass, and the CPU baseline (the oracle's threaded ECL-CC restatement on the
-host cores, same graph).
+CSR resident in HBM.  One step = one full CC pass (init ... flatten).  Prints
+ONE JSON line with edges/s (edges = CSR entries, the reference's definition,
+ECL-CC-atomic.cu:316), the HBM roofline of the pass, the CPU baseline (the
+oracle's ECL-CC restatement on the host cores, threaded and serial, same
+graph), and the other half of the metric as two objects of the same line:
+``batched`` (the 4,096-graph mix in one hipGraph launch: graphs/s against
+one graph per launch) and ``served`` (1,024 requests from the mix under
+Poisson arrivals: graphs/s, p50/p99 ms against one request at a time).
 
 Other modes: ``--mode batched`` (many graphs, one hipGraph launch, against
 one graph per launch), ``--mode served`` (continuous batching under
@@ -51,29 +54,58 @@ def make_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def cpu_share() -> tuple:
+    """Host cores this process may use: the CPUs in its affinity mask, capped
+    by the CPU share the host grants it (OMP_NUM_THREADS, when set), and
+    os.cpu_count() for reference."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(avail, int(share)) if share.isdigit() and int(share) > 0 else avail
+    return cores, avail, os.cpu_count() or 1, share or None
+
+
 def cpu_baseline(g, budget_s: float, gpu_labels):
-    """Threaded ECL-CC restatement (oracle/) on the host cores."""
+    """The oracle's ECL-CC restatement (ECL-CC-atomic.cu:69-264) on the host:
+    CPU-N threaded on every core this process may use, and CPU-1 serial, on
+    the full graph; each the median of as many runs as fit the budget (at
+    least one, at most 9)."""
     import numpy as np
     from oracle import oracle
 
-    cores = min(16, os.cpu_count() or 1)
-    times = []
-    t_end = time.perf_counter() + budget_s
-    labels = None
-    while len(times) < 9 and (not times or time.perf_counter() < t_end):
-        t0 = time.perf_counter()
-        labels, used = oracle.eclcc_threads(g.nindex, g.nlist, cores)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
+    cores, avail, host, share = cpu_share()
+
+    def timed(fn, budget):
+        times, out = [], None
+        t_end = time.perf_counter() + budget
+        while len(times) < 9 and (not times or time.perf_counter() < t_end):
+            t0 = time.perf_counter()
+            out = fn()
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times), len(times), out
+
+    medN, runsN, (labels, used) = timed(lambda: oracle.eclcc_threads(g.nindex, g.nlist, cores),
+                                        budget_s / 2)
+    med1, runs1, labels1 = timed(lambda: oracle.eclcc_serial(g.nindex, g.nlist), budget_s / 2)
     return {
-        "value": g.edges / med,
+        "value": g.edges / medN,
         "unit": "edges/s",
         "cores": used,
         "kind": "port",
-        "sample": f"full {g.name} (n={g.nodes}, m={g.edges}), median of {len(times)} runs of the "
-                  f"oracle's threaded ECL-CC restatement",
-        "seconds": med,
+        "sample": f"full {g.name} (n={g.nodes}, m={g.edges}): the oracle's ECL-CC restatement, "
+                  f"CPU-N on {used} threads (median of {runsN}) and CPU-1 serial (median of {runs1})",
+        "seconds": medN,
+        "host_cpus": host,
+        "affinity_cpus": avail,
+        "cpu_share_env": share,
         "labels_match_gpu": bool(np.array_equal(labels, gpu_labels)),
+        "cpu1": {
+            "value": g.edges / med1,
+            "unit": "edges/s",
+            "cores": 1,
+            "seconds": med1,
+            "runs": runs1,
+            "labels_match_gpu": bool(np.array_equal(labels1, gpu_labels)),
+        },
     }
 
 
@@ -218,7 +250,43 @@ def plain(args) -> dict:
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, args.cpu_budget, labels)
+    # the batched and served halves of the metric, on the graph mix
+    dg = g = labels = None
+    torch.cuda.empty_cache()
+    from eclcc import benchmodes
+
+    if not args.no_batched:
+        log("batched: the graph mix in one hipGraph launch")
+        out["batched"] = summary(benchmodes.batched(args, check=oracle_check))
+        torch.cuda.empty_cache()
+    if not args.no_served:
+        log("served: Poisson arrivals into the continuous-batching server")
+        out["served"] = summary(benchmodes.served(args, check=oracle_check))
+        torch.cuda.empty_cache()
     return out
+
+
+def oracle_check(graphs, labels) -> bool:
+    """Checker (never timed): every graph's labels equal the oracle's
+    minimum-ID labels, computed on the host cores this process may use."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from oracle import oracle
+
+    def one(i):
+        g = graphs[i]
+        return bool(np.array_equal(labels[i], oracle.cc_labels(g.nindex, g.nlist)))
+
+    with ThreadPoolExecutor(max_workers=cpu_share()[0]) as ex:
+        return all(ex.map(one, range(len(graphs))))
+
+
+def summary(d: dict) -> dict:
+    """A batched/served result as an object of the one bench line (the keys
+    the line already carries at its top level are dropped)."""
+    drop = {"metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data"}
+    return {k: v for k, v in d.items() if k not in drop}
 
 
 def racy(args) -> dict:
@@ -428,10 +496,15 @@ def main() -> None:
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the read-every-edge (giant_filter 1) comparison pass")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="plain mode: leave out the batched object of the line")
+    ap.add_argument("--no-served", action="store_true",
+                    help="plain mode: leave out the served object of the line")
     # batched / served
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--tune-graphs", type=int, default=256)
-    ap.add_argument("--tune-budget", type=float, default=60.0)
+    ap.add_argument("--tune-budget", type=float, default=0.0,
+                    help="seconds of autotuning on a sample of the mix; 0 = the committed table")
     ap.add_argument("--requests", type=int, default=1024)
     ap.add_argument("--qps", type=float, default=0.0, help="offered load; 0 = --load x capacity")
     ap.add_argument("--load", type=float, default=0.75)
@@ -456,7 +529,7 @@ def main() -> None:
     else:
         from eclcc import benchmodes
 
-        out = benchmodes.run(args.mode, args)
+        out = benchmodes.run(args.mode, args, check=oracle_check)
     print(json.dumps(out), flush=True)
 
 

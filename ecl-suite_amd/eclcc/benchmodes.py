@@ -45,7 +45,26 @@ def _labels_equal(a: List[np.ndarray], b: List[np.ndarray]) -> bool:
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-def batched(args) -> dict:
+def batch_params(args, graphs):
+    """The batch's parameters: the committed autotuner table (tuning/
+    mi355x.json, key batch_mix) unless a tuning budget is given, in which
+    case the autotuner runs afresh on a sample of the mix."""
+    if args.tune_budget > 0:
+        t0 = time.perf_counter()
+        sample = graphs[: min(args.tune_graphs, len(graphs))]
+        with Batch(sample) as sb:
+            best, trials = autotune.tune(sb, budget_s=args.tune_budget)
+        log(f"autotuned on {len(sample)} graphs in {time.perf_counter() - t0:.1f}s: {best}")
+        return default_params(**best), f"autotuned on {len(sample)} graphs ({len(trials)} trials)"
+    p = autotune.load("batch_mix")
+    if p is None:
+        return default_params(), "built-in defaults"
+    return p, "tuning/mi355x.json (batch_mix)"
+
+
+def batched(args, check=None) -> dict:
+    """check(graphs, labels) -> bool, if given, is called on the batched
+    run's labels outside the timed region."""
     import torch
 
     torch.cuda.set_device(0)
@@ -57,13 +76,8 @@ def batched(args) -> dict:
     log(f"generated {count} graphs: {n_tot} vertices, {m_tot} CSR entries in "
         f"{time.perf_counter() - t0:.1f}s")
 
-    # autotune on a sample of the mix
-    t0 = time.perf_counter()
-    sample = graphs[: min(args.tune_graphs, count)]
-    with Batch(sample) as sb:
-        best, trials = autotune.tune(sb, budget_s=args.tune_budget)
-    log(f"autotuned on {len(sample)} graphs in {time.perf_counter() - t0:.1f}s: {best}")
-    params = default_params(**best)
+    params, params_source = batch_params(args, graphs)
+    best = params.as_dict()
 
     t0 = time.perf_counter()
     b = Batch(graphs, params)
@@ -82,6 +96,10 @@ def batched(args) -> dict:
     ms = start.elapsed_time(end) / args.steps
     kms, _ = b.run_profiled(stream)
     lab_batch = b.all_labels()
+    oracle_ok = None
+    if check is not None:
+        log("labels against the oracle (outside the timed region)")
+        oracle_ok = bool(check(graphs, lab_batch))
 
     # one graph per launch, synchronising after each (the plain API's usage)
     log("one graph per launch")
@@ -127,7 +145,8 @@ def batched(args) -> dict:
         "speedup_vs_plain_sync": (count / (ms * 1e-3)) / (count / plain_sync_s),
         "speedup_vs_plain_async": (count / (ms * 1e-3)) / (count / plain_async_s),
         "labels_batched_equal_plain": bool(same),
-        "autotune_trials": len(trials),
+        "labels_equal_oracle": oracle_ok,
+        "params_source": params_source,
         "kernels_ms": kms,
     }
     b.close()
@@ -155,10 +174,12 @@ class PinnedPool:
             self.items.append(type(g)(ni, nl, None, g.name))
 
 
-def served(args) -> dict:
+def served(args, check=None) -> dict:
     """Requests name graphs resident in HBM (a graph store); the server writes
     each request's labels in place.  Latency = submit -> labels in HBM.  The
-    host-array path (CSR copied in, labels copied out) is measured beside it."""
+    host-array path (CSR copied in, labels copied out) is measured beside it.
+    check(graphs, labels) -> bool, if given, is called on the served labels
+    after the timed runs."""
     import torch
 
     from .server import Server
@@ -278,6 +299,10 @@ def served(args) -> dict:
         span = max(done_t) - t0
         achieved_qps = R / (submit_t[-1] - t0) if R > 1 else qps
         same = same and check_all()
+        oracle_ok = None
+        if check is not None:
+            log("served labels against the oracle (after the timed runs)")
+            oracle_ok = bool(check(graphs, [lab.cpu().numpy() for (_, _, _, _, lab) in store]))
         st = srv.stats()
         st["poisson_batches"] = st["batches"] - b0
 
@@ -337,6 +362,7 @@ def served(args) -> dict:
         "speedup_served_vs_plain": served_gps / plain_gps,
         "speedup_capacity_vs_plain": cap_gps / plain_gps,
         "labels_equal_plain": bool(same and host_ok),
+        "labels_equal_oracle": oracle_ok,
         "server_stats": st,
         "host_requests": {
             "capacity_graphs_per_s": host_cap_gps,
@@ -348,9 +374,9 @@ def served(args) -> dict:
     }
 
 
-def run(mode: str, args) -> dict:
+def run(mode: str, args, check=None) -> dict:
     if mode == "batched":
-        return batched(args)
+        return batched(args, check)
     if mode == "served":
-        return served(args)
+        return served(args, check)
     raise ValueError(mode)
