@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -445,48 +446,214 @@ int eclgraph_from_edges(int32_t nodes, int64_t nedges, const int32_t* src, const
   return host_exception();
 }
 
+// ---- MatrixMarket conversion (scripts/mm2ecl_real.cpp:50-184) ----------------
+
+namespace {
+
+// One entry of a coordinate file: 0-based row and column, weight.  Real
+// weights are kept as read (double) until the CSR is written, because the
+// reference sorts by the value before scaling it (mm2ecl_real.cpp:153-172).
+struct MtxEntry {
+  int32_t src, dst;
+  int64_t iw;  // integer weight
+  double fw;   // real weight
+};
+
+// Scanning of the entry section the way the reference's fscanf loop reads
+// it (:98, :124, :154): whitespace, newlines included, separates numbers
+// anywhere; "%d" is an optional sign and at least one decimal digit, "%lf"
+// whatever strtod takes.  The loop stops at the first entry that does not
+// scan completely, and the entry count is checked afterwards.
+struct Scanner {
+  const char* p;
+  const char* end;
+  void skip_ws() {
+    while (p < end && std::isspace((unsigned char)*p)) p++;
+  }
+  bool int_token(int64_t* v) {
+    skip_ws();
+    const char* q = p;
+    bool neg = false;
+    if (q < end && (*q == '+' || *q == '-')) neg = *q++ == '-';
+    if (q >= end || !std::isdigit((unsigned char)*q)) return false;
+    int64_t x = 0;
+    while (q < end && std::isdigit((unsigned char)*q)) {
+      if (x < (int64_t)1 << 40) x = x * 10 + (*q - '0');  // clamps far past int32
+      q++;
+    }
+    *v = neg ? -x : x;
+    p = q;
+    return true;
+  }
+  bool real_token(double* v) {
+    skip_ws();
+    if (p >= end) return false;
+    char* stop = nullptr;
+    *v = std::strtod(p, &stop);  // the buffer ends in a NUL, so strtod stops there
+    if (stop == p) return false;
+    p = stop;
+    return true;
+  }
+};
+
+}  // namespace
+
+int eclgraph_mm2ecl(const char* path, eclgraph_csr** out, int* weight_kind) try {
+  if (!path || !out) return fail(ECLGRAPH_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (weight_kind) *weight_kind = ECLGRAPH_WEIGHTS_NONE;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(ECLGRAPH_ERR_IO, std::string("could not open input file ") + path);
+  std::string text;
+  {
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    if (std::ferror(f)) return fail(ECLGRAPH_ERR_IO, std::string("could not read ") + path);
+  }
+  // lines as getline returns them (each with its newline, if it has one)
+  size_t pos = 0;
+  auto next_line = [&](std::string* line) {
+    if (pos >= text.size()) return false;
+    const size_t nl = text.find('\n', pos);
+    const size_t stop = nl == std::string::npos ? text.size() : nl + 1;
+    line->assign(text, pos, stop - pos);
+    pos = stop;
+    return true;
+  };
+  auto has = [](const std::string& line, const char* word) {
+    return line.find(word) != std::string::npos;
+  };
+  std::string line;
+  // banner (:62-72): at least 30 characters, and these words, case-sensitive
+  if (!next_line(&line) || line.size() < 30) return fail(ECLGRAPH_ERR_FORMAT, "could not read first line");
+  if (!has(line, "%%MatrixMarket"))
+    return fail(ECLGRAPH_ERR_FORMAT, "first line does not contain \"%%MatrixMarket\"");
+  if (!has(line, "matrix")) return fail(ECLGRAPH_ERR_FORMAT, "first line does not contain \"matrix\"");
+  if (!has(line, "coordinate"))
+    return fail(ECLGRAPH_ERR_FORMAT, "first line does not contain \"coordinate\"");
+  if (!has(line, "general") && !has(line, "symmetric"))
+    return fail(ECLGRAPH_ERR_FORMAT, "first line does not contain \"general\" or \"symmetric\"");
+  if (!has(line, "integer") && !has(line, "pattern") && !has(line, "real"))
+    return fail(ECLGRAPH_ERR_FORMAT,
+                "first line does not contain \"integer\" or \"pattern\" or \"real\"");
+  // integer takes precedence over real, as in the reference (:70-72, :93-145)
+  const int kind = has(line, "integer") ? ECLGRAPH_WEIGHTS_INT
+                   : has(line, "real")  ? ECLGRAPH_WEIGHTS_REAL
+                                        : ECLGRAPH_WEIGHTS_NONE;
+  // comment lines: every line holding a '%' anywhere (:74); then the size line
+  bool got = false;
+  while ((got = next_line(&line)) && has(line, "%")) {
+  }
+  if (!got || line.size() < 3) return fail(ECLGRAPH_ERR_FORMAT, "could not find non-comment line");
+  long long rows = 0, cols = 0, nnz = 0;
+  if (std::sscanf(line.c_str(), "%lld %lld %lld", &rows, &cols, &nnz) != 3 || rows < 1 || nnz < 0 ||
+      rows != cols)
+    return fail(ECLGRAPH_ERR_FORMAT, "failed to parse first data line");
+  if (rows > INT32_MAX || nnz > INT32_MAX)
+    return fail(ECLGRAPH_ERR_RANGE, "matrix size exceeds the int32 format");
+  const int32_t nodes = (int32_t)rows;
+  // entries, scanned from the rest of the text
+  Scanner sc{text.c_str() + pos, text.c_str() + text.size()};
+  std::vector<MtxEntry> ent;
+  ent.reserve((size_t)nnz);
+  for (;;) {
+    int64_t i, j, w = 0;
+    double x = 0.0;
+    if (!sc.int_token(&i) || !sc.int_token(&j)) break;
+    if (kind == ECLGRAPH_WEIGHTS_INT && !sc.int_token(&w)) break;
+    if (kind == ECLGRAPH_WEIGHTS_REAL && !sc.real_token(&x)) break;
+    if (i < 1 || i > nodes || j < 1 || j > nodes)
+      return fail(ECLGRAPH_ERR_FORMAT, "source out of range");
+    if (kind == ECLGRAPH_WEIGHTS_INT && (w < INT32_MIN || w > INT32_MAX))
+      return fail(ECLGRAPH_ERR_RANGE, "integer weight exceeds the int32 format");
+    if (kind == ECLGRAPH_WEIGHTS_REAL && !(std::fabs(x * 100.0) < 2147483648.0))
+      return fail(ECLGRAPH_ERR_RANGE, "real weight x 100 exceeds the int32 format");
+    if ((long long)ent.size() >= nnz)
+      return fail(ECLGRAPH_ERR_FORMAT, "failed to read correct number of edges");
+    ent.push_back(MtxEntry{(int32_t)(i - 1), (int32_t)(j - 1), w, x});
+  }
+  if ((long long)ent.size() != nnz) return fail(ECLGRAPH_ERR_FORMAT, "failed to read correct number of edges");
+  // order by (row, column, weight) as the reference's tuple sort does
+  std::sort(ent.begin(), ent.end(), [kind](const MtxEntry& a, const MtxEntry& b) {
+    if (a.src != b.src) return a.src < b.src;
+    if (a.dst != b.dst) return a.dst < b.dst;
+    return kind == ECLGRAPH_WEIGHTS_REAL ? a.fw < b.fw : a.iw < b.iw;
+  });
+  std::unique_ptr<eclgraph_csr> g(new (std::nothrow) eclgraph_csr);
+  if (!g) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory");
+  g->nodes = nodes;
+  g->edges = (int64_t)ent.size();
+  g->nindex = alloc_array<int32_t>((int64_t)nodes + 1);
+  g->nlist = alloc_array<int32_t>(g->edges);
+  if (!g->nindex || !g->nlist) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (CSR)");
+  if (kind != ECLGRAPH_WEIGHTS_NONE) {
+    g->eweight = alloc_array<int32_t>(g->edges);
+    if (!g->eweight) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (weights)");
+  }
+  // nindex[v + 1] = entries whose row is at most v
+  std::fill(g->nindex.get(), g->nindex.get() + nodes + 1, 0);
+  for (const MtxEntry& e : ent) g->nindex[e.src + 1]++;
+  for (int32_t v = 0; v < nodes; v++) g->nindex[v + 1] += g->nindex[v];
+  for (size_t k = 0; k < ent.size(); k++) {
+    g->nlist[k] = ent[k].dst;
+    if (kind == ECLGRAPH_WEIGHTS_INT) g->eweight[k] = (int32_t)ent[k].iw;
+    // scaled and truncated toward zero (:172)
+    if (kind == ECLGRAPH_WEIGHTS_REAL) g->eweight[k] = (int32_t)(ent[k].fw * 100.0);
+  }
+  if (weight_kind) *weight_kind = kind;
+  *out = g.release();
+  return ECLGRAPH_OK;
+} catch (...) {
+  return host_exception();
+}
+
 int eclgraph_read_mtx(const char* path, int flags, eclgraph_csr** out) try {
   if (!path || !out) return fail(ECLGRAPH_ERR_ARG, "null argument");
   *out = nullptr;
-  FILE* f = std::fopen(path, "r");
-  if (!f) return fail(ECLGRAPH_ERR_IO, std::string("could not open file ") + path);
-  std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
-  char line[4096];
-  if (!std::fgets(line, sizeof line, f)) return fail(ECLGRAPH_ERR_FORMAT, "empty MatrixMarket file");
-  std::string hdr(line);
-  for (auto& ch : hdr) ch = (char)std::tolower((unsigned char)ch);
-  if (hdr.rfind("%%matrixmarket", 0) != 0 || hdr.find("coordinate") == std::string::npos)
-    return fail(ECLGRAPH_ERR_FORMAT, "not a MatrixMarket coordinate file");
-  const bool mirror = hdr.find("symmetric") != std::string::npos ||
-                      hdr.find("hermitian") != std::string::npos ||
-                      hdr.find("skew") != std::string::npos;
-  long long rows = 0, cols = 0, nnz = 0;
-  for (;;) {
-    if (!std::fgets(line, sizeof line, f)) return fail(ECLGRAPH_ERR_FORMAT, "missing size line");
-    if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
-    if (std::sscanf(line, "%lld %lld %lld", &rows, &cols, &nnz) != 3)
-      return fail(ECLGRAPH_ERR_FORMAT, "bad size line");
-    break;
+  eclgraph_csr* raw = nullptr;
+  int rc = eclgraph_mm2ecl(path, &raw, nullptr);
+  if (rc) return rc;
+  std::unique_ptr<eclgraph_csr> g(raw);
+  if (!(flags & (ECLGRAPH_SYMMETRIZE | ECLGRAPH_DROP_SELF | ECLGRAPH_DEDUPE))) {
+    *out = g.release();
+    return ECLGRAPH_OK;
   }
-  const long long n = std::max(rows, cols);
-  if (n < 1 || n > INT32_MAX || nnz < 0) return fail(ECLGRAPH_ERR_RANGE, "matrix size out of range");
-  std::vector<int32_t> s, d;
-  s.reserve((size_t)nnz * (mirror ? 2 : 1));
-  d.reserve((size_t)nnz * (mirror ? 2 : 1));
-  for (long long k = 0; k < nnz; k++) {
-    if (!std::fgets(line, sizeof line, f)) return fail(ECLGRAPH_ERR_FORMAT, "truncated entry list");
-    if (line[0] == '%') { k--; continue; }
-    long long i, j;
-    if (std::sscanf(line, "%lld %lld", &i, &j) != 2) return fail(ECLGRAPH_ERR_FORMAT, "bad entry line");
-    if (i < 1 || j < 1 || i > n || j > n) return fail(ECLGRAPH_ERR_RANGE, "entry index out of range");
-    s.push_back((int32_t)(i - 1));
-    d.push_back((int32_t)(j - 1));
-    if (mirror && i != j) {
-      s.push_back((int32_t)(j - 1));
-      d.push_back((int32_t)(i - 1));
+  // (row, column, weight) triples; mirrored, filtered, sorted, deduplicated
+  struct T {
+    int32_t u, v, w;
+  };
+  const bool wt = (bool)g->eweight;
+  std::vector<T> t;
+  t.reserve((size_t)g->edges * ((flags & ECLGRAPH_SYMMETRIZE) ? 2 : 1));
+  for (int32_t u = 0; u < g->nodes; u++)
+    for (int32_t k = g->nindex[u]; k < g->nindex[u + 1]; k++) {
+      const int32_t v = g->nlist[k], w = wt ? g->eweight[k] : 0;
+      if ((flags & ECLGRAPH_DROP_SELF) && u == v) continue;
+      t.push_back(T{u, v, w});
+      if ((flags & ECLGRAPH_SYMMETRIZE) && u != v) t.push_back(T{v, u, w});
     }
+  std::sort(t.begin(), t.end(), [](const T& a, const T& b) {
+    return a.u != b.u ? a.u < b.u : a.v != b.v ? a.v < b.v : a.w < b.w;
+  });
+  if (flags & ECLGRAPH_DEDUPE)  // keeps the lightest of equal (row, column)
+    t.erase(std::unique(t.begin(), t.end(), [](const T& a, const T& b) { return a.u == b.u && a.v == b.v; }),
+            t.end());
+  if ((int64_t)t.size() > INT32_MAX) return fail(ECLGRAPH_ERR_RANGE, "edge count exceeds int32 CSR format");
+  g->edges = (int64_t)t.size();
+  g->nlist = alloc_array<int32_t>(g->edges);
+  if (wt) g->eweight = alloc_array<int32_t>(g->edges);
+  if (!g->nlist || (wt && !g->eweight)) return fail(ECLGRAPH_ERR_ALLOC, "out of host memory (CSR)");
+  std::fill(g->nindex.get(), g->nindex.get() + g->nodes + 1, 0);
+  for (size_t k = 0; k < t.size(); k++) {
+    g->nindex[t[k].u + 1]++;
+    g->nlist[k] = t[k].v;
+    if (wt) g->eweight[k] = t[k].w;
   }
-  return build_csr((int32_t)n, (int64_t)s.size(), PairArrays{s.data(), d.data()}, flags, out);
+  for (int32_t v = 0; v < g->nodes; v++) g->nindex[v + 1] += g->nindex[v];
+  *out = g.release();
+  return ECLGRAPH_OK;
 } catch (...) {
   return host_exception();
 }

@@ -84,6 +84,7 @@ def graph_lib() -> C.CDLL:
         L.eclgraph_from_edges.argtypes = [i32, i64, vp, vp, C.c_int, C.POINTER(H)]
         L.eclgraph_check.argtypes = [i32, i64, vp, vp, C.c_int, C.POINTER(i64)]
         L.eclgraph_read_mtx.argtypes = [C.c_char_p, C.c_int, C.POINTER(H)]
+        L.eclgraph_mm2ecl.argtypes = [C.c_char_p, C.POINTER(H), C.POINTER(C.c_int)]
         L.eclgraph_nodes.argtypes = [H]
         L.eclgraph_nodes.restype = i32
         L.eclgraph_edges.argtypes = [H]

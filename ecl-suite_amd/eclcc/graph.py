@@ -142,8 +142,22 @@ class Graph:
         return Graph._from_handle(h, name)
 
     @staticmethod
-    def read_mtx(path: str, flags: int = SYMMETRIZE | DROP_SELF | DEDUPE) -> "Graph":
-        """MatrixMarket coordinate file (scripts/mm2ecl_real.cpp:50-184)."""
+    def mm2ecl(path: str):
+        """MatrixMarket coordinate file -> the CSR the reference's converter
+        writes (scripts/mm2ecl_real.cpp:50-184): entries as given, ordered by
+        (row, column, weight), integer weights kept, real weights x 100
+        truncated.  Returns (graph, kind) with kind "no", "int" or "float"
+        as the reference prints it."""
+        h = C.c_void_p()
+        kind = C.c_int(0)
+        N.check_graph(N.graph_lib().eclgraph_mm2ecl(str(path).encode(), C.byref(h), C.byref(kind)))
+        return Graph._from_handle(h, str(path)), ("no", "int", "float")[kind.value]
+
+    @staticmethod
+    def read_mtx(path: str, flags: int = 0) -> "Graph":
+        """mm2ecl followed by the from_edges flags on the weighted entries:
+        flags = 0 is the reference's conversion; SYMMETRIZE | DROP_SELF |
+        DEDUPE gives the symmetric CSR the CC path needs."""
         h = C.c_void_p()
         N.check_graph(N.graph_lib().eclgraph_read_mtx(str(path).encode(), int(flags), C.byref(h)))
         return Graph._from_handle(h, str(path))

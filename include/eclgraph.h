@@ -93,9 +93,35 @@ enum { ECLGRAPH_SYMMETRIZE = 1, ECLGRAPH_DROP_SELF = 2, ECLGRAPH_DEDUPE = 4 };
 int eclgraph_from_edges(int32_t nodes, int64_t nedges, const int32_t* src,
                         const int32_t* dst, int flags, eclgraph_csr** out);
 
-/* MatrixMarket coordinate file -> CSR (the job of scripts/mm2ecl_real.cpp,
-   :50-184). Pattern and real/integer matrices; "symmetric" files are
-   expanded. flags as for eclgraph_from_edges. Weights are not kept. */
+/* Weights of a converted MatrixMarket file. */
+enum { ECLGRAPH_WEIGHTS_NONE = 0, ECLGRAPH_WEIGHTS_INT = 1, ECLGRAPH_WEIGHTS_REAL = 2 };
+
+/* Replaces scripts/mm2ecl_real.cpp (:50-184): the MatrixMarket coordinate
+   file at `path` becomes the CSR the reference's converter writes, byte for
+   byte once written with eclgraph_write:
+     - the banner line holds at least 30 characters and the words
+       "%%MatrixMarket", "matrix", "coordinate", "general" or "symmetric",
+       and "integer", "pattern" or "real" (case-sensitive);
+     - every following line that holds a '%' is a comment; the size line
+       must be square (rows == columns >= 1, entries >= 0);
+     - entries are kept as given: a "symmetric" file is NOT mirrored,
+       duplicates and self-loops stay; indices must lie in [1, rows] and
+       their count must equal the size line's;
+     - entries are ordered by (row, column, weight);
+     - "integer" files keep their weights, "real" files store
+       (int)(weight * 100) (truncated toward zero), "pattern" files have
+       none.  *weight_kind (optional) receives ECLGRAPH_WEIGHTS_*.
+   Failures the reference reports with exit(-1) are ECLGRAPH_ERR_FORMAT /
+   ERR_IO; values it leaves undefined (an int32 overflow) are
+   ECLGRAPH_ERR_RANGE. */
+int eclgraph_mm2ecl(const char* path, eclgraph_csr** out, int* weight_kind);
+
+/* eclgraph_mm2ecl followed by the flags of eclgraph_from_edges on the
+   (row, column, weight) entries: ECLGRAPH_SYMMETRIZE adds each reverse
+   entry with the same weight, ECLGRAPH_DROP_SELF drops self-loops,
+   ECLGRAPH_DEDUPE keeps the lightest of equal (row, column) entries.
+   flags = 0 is the reference's conversion; ECL-CC needs the symmetric
+   CSR that SYMMETRIZE | DROP_SELF | DEDUPE gives.  Weights are kept. */
 int eclgraph_read_mtx(const char* path, int flags, eclgraph_csr** out);
 
 /* ---- seeded synthetic inputs (the reference downloads its inputs,

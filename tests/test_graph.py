@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from eclcc import EclgraphError, Graph, workloads
-from eclcc.graph import rand64, set_generator_threads
+from eclcc.graph import DEDUPE, DROP_SELF, SYMMETRIZE, rand64, set_generator_threads
 from oracle import oracle
 
 
@@ -140,15 +140,17 @@ def test_mtx_reader(tmp_path):
     p = tmp_path / "m.mtx"
     p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n"
                  "5 5 4\n2 1\n3 2\n5 4\n4 4\n")
-    g = Graph.read_mtx(p)
+    g = Graph.read_mtx(p, SYMMETRIZE | DROP_SELF | DEDUPE)
     assert g.nodes == 5
     assert_clean_symmetric(g)
     assert oracle.cc_labels(g.nindex, g.nlist).tolist() == [0, 0, 0, 3, 3]
     q = tmp_path / "r.mtx"
-    q.write_text("%%MatrixMarket matrix coordinate real general\n3 4 2\n1 4 2.5\n3 2 1e3\n")
-    h = Graph.read_mtx(q)
+    q.write_text("%%MatrixMarket matrix coordinate real general\n4 4 2\n1 4 2.5\n3 2 1e3\n")
+    h = Graph.read_mtx(q, SYMMETRIZE | DROP_SELF | DEDUPE)
     assert h.nodes == 4
     assert_clean_symmetric(h)
+    # the weights travel with their entries: (0,3) and (3,0) weigh 250
+    assert h.eweight.tolist() == [250, 100000, 100000, 250]
 
 
 def test_from_edges_flags():
