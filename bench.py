@@ -255,13 +255,13 @@ def plain(args) -> dict:
     torch.cuda.empty_cache()
     from eclcc import benchmodes
 
-    if not args.no_batched:
-        log("batched: the graph mix in one hipGraph launch")
-        out["batched"] = summary(benchmodes.batched(args, check=oracle_check))
-        torch.cuda.empty_cache()
     if not args.no_served:
         log("served: Poisson arrivals into the continuous-batching server")
         out["served"] = summary(benchmodes.served(args, check=oracle_check))
+        torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("batched: the graph mix in one hipGraph launch")
+        out["batched"] = summary(benchmodes.batched(args, check=oracle_check))
         torch.cuda.empty_cache()
     return out
 
@@ -510,8 +510,11 @@ def main() -> None:
     ap.add_argument("--load", type=float, default=0.75)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
-    ap.add_argument("--serve-streams", type=int, default=4)
-    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-streams", type=int, default=3)
+    ap.add_argument("--serve-max-batch", type=int, default=128)
+    ap.add_argument("--serve-wait-us", type=int, default=100,
+                    help="a batch smaller than --serve-max-batch waits this long for company "
+                         "while another batch is on the GPU")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -273,6 +273,26 @@ void eclcc_params_default(eclcc_params* p) {
   p->path_mode = 1;
 }
 
+int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, eclcc_params* p) {
+  if (!p) return set_error(ECLCC_ERR_ARG, "null params");
+  eclcc_params_default(p);
+  if (nodes < 1 || edges < 0 || !nindex) return set_error(ECLCC_ERR_ARG, "bad graph arguments");
+  // the largest of (up to) 4096 evenly spaced degrees against the average
+  const int32_t samples = nodes < 4096 ? nodes : 4096;
+  long long dmax = 0;
+  for (int32_t k = 0; k < samples; k++) {
+    const int32_t v = (int32_t)(((long long)k * nodes) / samples);
+    dmax = std::max<long long>(dmax, (long long)nindex[v + 1] - nindex[v]);
+  }
+  const double avg = (double)edges / (double)nodes;
+  // every list about as long as the average: the vertices a sampled hook
+  // leaves outside the largest tree have short lists, which compute1 takes
+  // cheaply, so half of the sampled hooks suffice; with a heavy tail they
+  // include long lists, and every sampled hook pays for itself
+  if ((double)dmax <= 4.0 * avg + 8.0) p->sample_shift = 1;
+  return ECLCC_OK;
+}
+
 size_t eclcc_workspace_bytes(int32_t nodes, int64_t edges, const eclcc_params* params) try {
   Tuning t;
   if (nodes < 1 || edges < 0 || resolve_tuning(params, &t) != ECLCC_OK) return 0;
@@ -378,6 +398,11 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
   int rc = check_csr_args(nodes, edges, nindex, nlist);
   if (rc) return rc;
   if (!nstat) return set_error(ECLCC_ERR_ARG, "null label pointer");
+  eclcc_params chosen;
+  if (!params) {
+    if ((rc = eclcc_params_for_graph(nodes, edges, nindex, &chosen))) return rc;
+    params = &chosen;
+  }
   Tuning t;
   if ((rc = resolve_tuning(params, &t))) return rc;
   int ndev = 0;

@@ -278,7 +278,15 @@ struct eclcc_server {
       {
         std::unique_lock<std::mutex> lock(mu);
         if (stop && pending.empty() && !busy) return;
-        if (free_k >= 0 && !pending.empty()) {
+        // while other batches run, a small batch waits for company until its
+        // oldest request has waited batch_wait_us: fuller batches carry less
+        // fixed cost per graph (an idle GPU takes whatever is waiting at once)
+        bool fill = free_k >= 0 && !pending.empty();
+        if (fill && busy && cfg.batch_wait_us > 0 && (int)pending.size() < cfg.max_batch) {
+          const auto waited = Clock::now() - pending.front()->t_submit;
+          fill = waited >= std::chrono::microseconds(cfg.batch_wait_us);
+        }
+        if (fill) {
           // the batch and its request list are allocated before any request
           // leaves the queue; if the host is out of memory the requests wait
           nb = new (std::nothrow) Batch;
@@ -384,6 +392,7 @@ int eclcc_server_create(const eclcc_server_config* cfg, const eclcc_params* para
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 4096;
   if (s->cfg.streams <= 0) s->cfg.streams = 4;
   if (s->cfg.max_batch <= 0) s->cfg.max_batch = 1024;
+  if (s->cfg.batch_wait_us < 0) return set_error(ECLCC_ERR_ARG, "batch_wait_us must be >= 0");
   size_t freeb = 0, total = 0;
   if ((rc = check_hip(hipMemGetInfo(&freeb, &total), ECLCC_ERR_DEVICE, "hipMemGetInfo failed"))) return rc;
   size_t budget = s->cfg.hbm_budget_bytes ? s->cfg.hbm_budget_bytes : freeb / 10 * 8;

@@ -8,10 +8,12 @@ HIP library libeclcc.so; importing works without a GPU, calling needs one.
 """
 from ._native import EclccError, EclccParams, EclgraphError, NativeLibraryMissing
 from .graph import Graph
-from .cc import DeviceGraph, compute, compute_cc, compute_timed, default_params, version
+from .cc import (DeviceGraph, compute, compute_cc, compute_timed, default_params,
+                 params_for_graph, version)
 from . import roofline, workloads
 
 __all__ = [
     "Graph", "DeviceGraph", "compute", "compute_cc", "compute_timed", "default_params",
+    "params_for_graph",
     "version", "workloads", "EclccError", "EclgraphError", "EclccParams", "NativeLibraryMissing",
 ]

@@ -50,7 +50,8 @@ class EclccServerConfig(C.Structure):
         ("max_inflight", i32),
         ("streams", i32),
         ("max_batch", i32),
-        ("reserved", i32 * 5),
+        ("batch_wait_us", i32),
+        ("reserved", i32 * 4),
     ]
 
 
@@ -115,6 +116,7 @@ def _declare_cc(L: C.CDLL) -> C.CDLL:
     L.eclcc_kernel_name.restype = C.c_char_p
     L.eclcc_params_default.argtypes = [PP]
     L.eclcc_params_default.restype = None
+    L.eclcc_params_for_graph.argtypes = [i32, i64, vp, PP]
     L.eclcc_compute.argtypes = [i32, i32, vp, vp, vp]
     L.eclcc_compute_timed.argtypes = [i32, i32, vp, vp, vp, PP, C.POINTER(C.c_double)]
     L.eclcc_workspace_bytes.argtypes = [i32, i64, PP]

@@ -17,6 +17,7 @@ copy/free pattern.
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import statistics
 import sys
 import time
@@ -240,7 +241,7 @@ def served(args, check=None) -> dict:
 
     budget = int(args.serve_budget_gib * 2**30)
     srv = Server(hbm_budget_bytes=budget, streams=args.serve_streams, max_batch=args.serve_max_batch,
-                 params=params)
+                 params=params, batch_wait_us=args.serve_wait_us)
     try:
         def check_all():
             return all(np.array_equal(lab.cpu().numpy(), ref)
@@ -261,50 +262,66 @@ def served(args, check=None) -> dict:
         log(f"server burst capacity {[round(c) for c in caps]} graphs/s")
         cap_gps = statistics.median(caps)
 
-        # Poisson arrivals at a fraction of the measured capacity; requests are
-        # submitted through the C ABI directly so that the producer keeps up
+        # Poisson arrivals at a fraction of the measured capacity, three
+        # independent seeded streams; requests are submitted through the C ABI
+        # directly so that the producer keeps up.  The run with the median
+        # sustained rate is reported, all three beside it.
         qps = args.qps if args.qps > 0 else args.load * cap_gps
-        rng = np.random.default_rng(args.seed)
-        arrivals = np.cumsum(rng.exponential(1.0 / qps, R))
-        log(f"server Poisson run at {qps:.0f} requests/s")
-        lab_all.fill_(-1)
-        torch.cuda.synchronize()
-        submit_t = [0.0] * R
-        rids = [0] * R
         sub = N.cc_lib().eclcc_server_submit_device
         h = srv._h
         argv = [(n, m, ni.data_ptr(), nl.data_ptr() if m else None, lab.data_ptr())
                 for (n, m, ni, nl, lab) in store]
 
-        # the producer spins on the clock (a sleep's wake-up granularity is
-        # coarser than the gaps between arrivals) and submits each request
-        # when its arrival time comes
-        b0 = srv.stats()["batches"]
-        t0 = time.perf_counter()
-        for i in range(R):
-            target = t0 + arrivals[i]
-            while time.perf_counter() < target:
-                pass
-            submit_t[i] = time.perf_counter()
-            rids[i] = sub(h, *argv[i])
-        for r in rids:
-            N.check_cc(int(r))
-        lats = []
-        done_t = []
-        lat = C.c_double(0.0)
-        for i in range(R):
-            N.check_cc(N.cc_lib().eclcc_server_wait(h, int(rids[i]), -1, C.byref(lat)))
-            lats.append(lat.value)
-            done_t.append(submit_t[i] + lat.value * 1e-3)
-        span = max(done_t) - t0
-        achieved_qps = R / (submit_t[-1] - t0) if R > 1 else qps
+        def poisson_run(seed):
+            arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / qps, R))
+            lab_all.fill_(-1)
+            torch.cuda.synchronize()
+            gc.collect()
+            submit_t = [0.0] * R
+            rids = [0] * R
+            b0 = srv.stats()["batches"]
+            # the producer spins on the clock (a sleep's wake-up granularity is
+            # coarser than the gaps between arrivals) and submits each request
+            # when its arrival time comes
+            t0 = time.perf_counter()
+            for i in range(R):
+                target = t0 + arrivals[i]
+                while time.perf_counter() < target:
+                    pass
+                submit_t[i] = time.perf_counter()
+                rids[i] = sub(h, *argv[i])
+            for r in rids:
+                N.check_cc(int(r))
+            lats, done_t = [], []
+            lat = C.c_double(0.0)
+            for i in range(R):
+                N.check_cc(N.cc_lib().eclcc_server_wait(h, int(rids[i]), -1, C.byref(lat)))
+                lats.append(lat.value)
+                done_t.append(submit_t[i] + lat.value * 1e-3)
+            span = max(done_t) - t0
+            return {"seed": seed, "graphs_per_s": R / span, "span_s": span,
+                    "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
+                    "mean_ms": float(np.mean(lats)),
+                    "achieved_arrival_rate": R / (submit_t[-1] - t0) if R > 1 else qps,
+                    "batches": srv.stats()["batches"] - b0, "labels_equal_plain": check_all()}
+
+        runs = []
+        for k in range(3):
+            log(f"server Poisson run {k + 1}/3 at {qps:.0f} requests/s")
+            runs.append(poisson_run(args.seed + k))
+            log(f"  {runs[-1]['graphs_per_s']:.0f} graphs/s, p50 {runs[-1]['p50_ms']:.3f} ms, "
+                f"p99 {runs[-1]['p99_ms']:.3f} ms")
+        med = sorted(runs, key=lambda r: r["graphs_per_s"])[1]
+        same = same and all(r["labels_equal_plain"] for r in runs)
+        span = med["span_s"]
+        achieved_qps = med["achieved_arrival_rate"]
         same = same and check_all()
         oracle_ok = None
         if check is not None:
             log("served labels against the oracle (after the timed runs)")
             oracle_ok = bool(check(graphs, [lab.cpu().numpy() for (_, _, _, _, lab) in store]))
         st = srv.stats()
-        st["poisson_batches"] = st["batches"] - b0
+        st["poisson_batches"] = med["batches"]
 
         # host-array requests: CSR copied in, labels copied out (PCIe path)
         log("server burst, host-array requests")
@@ -350,10 +367,12 @@ def served(args, check=None) -> dict:
                    "qps": qps, "load": qps / cap_gps, "achieved_arrival_rate": achieved_qps,
                    "hbm_budget_bytes": budget,
                    "streams": args.serve_streams, "max_batch": args.serve_max_batch,
+                   "batch_wait_us": args.serve_wait_us,
                    "params": params.as_dict()},
-        "p50_ms": float(np.percentile(lats, 50)),
-        "p99_ms": float(np.percentile(lats, 99)),
-        "mean_ms": float(np.mean(lats)),
+        "p50_ms": med["p50_ms"],
+        "p99_ms": med["p99_ms"],
+        "mean_ms": med["mean_ms"],
+        "poisson_runs": runs,
         "capacity_graphs_per_s": cap_gps,
         "capacity_bursts_graphs_per_s": caps,
         "plain_graphs_per_s": plain_gps,

@@ -33,6 +33,19 @@ def default_params(**overrides) -> N.EclccParams:
     return p
 
 
+def params_for_graph(g: Graph, **overrides) -> N.EclccParams:
+    """The defaults tuned for g's class (eclcc_params_for_graph: degree
+    statistics of the host CSR), then `overrides`."""
+    p = N.EclccParams()
+    ni = np.ascontiguousarray(g.nindex, dtype=np.int32)
+    N.check_cc(N.cc_lib().eclcc_params_for_graph(g.nodes, g.edges, ni.ctypes.data, C.byref(p)))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, int(v))
+    return p
+
+
 def _pp(params: Optional[N.EclccParams]):
     return None if params is None else C.byref(params)
 
@@ -77,7 +90,7 @@ class DeviceGraph:
 
         self.torch = torch
         self.device = torch.device(device)
-        self.params = params if params is not None else default_params()
+        self.params = params if params is not None else params_for_graph(g)
         self.nodes, self.edges = g.nodes, g.edges
         self.nindex = torch.from_numpy(np.ascontiguousarray(g.nindex)).to(self.device)
         self.nlist = torch.from_numpy(np.ascontiguousarray(g.nlist)).to(self.device) if g.edges else \

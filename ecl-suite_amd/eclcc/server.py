@@ -19,12 +19,14 @@ from .graph import Graph
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 0, max_inflight: int = 0, streams: int = 0,
-                 max_batch: int = 0, params: Optional[N.EclccParams] = None):
+                 max_batch: int = 0, params: Optional[N.EclccParams] = None,
+                 batch_wait_us: int = 0):
         cfg = N.EclccServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_inflight = int(max_inflight)
         cfg.streams = int(streams)
         cfg.max_batch = int(max_batch)
+        cfg.batch_wait_us = int(batch_wait_us)
         h = C.c_void_p()
         N.check_cc(N.cc_lib().eclcc_server_create(C.byref(cfg),
                                                   None if params is None else C.byref(params),

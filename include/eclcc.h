@@ -84,6 +84,17 @@ typedef struct eclcc_params {
 /* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */
 void eclcc_params_default(eclcc_params* p);
 
+/* Fills *p with the defaults tuned for this graph's class, chosen on the
+   host from its degree statistics: graphs whose lists all have about the
+   same length (3-D grids, Erdos-Renyi: the largest of 4096 evenly spaced
+   degrees is at most 4 x the average + 8) take the sampled second-entry
+   hooks of half of their vertices (sample_shift 1; grid 256^3: 1.09 -> 0.97
+   ms per pass on MI355X); skewed degree distributions (R-MAT, power-law)
+   keep them all (sample_shift 0; R-MAT-24: 0.50 ms, 0.59 with 1).  nindex
+   is host memory; at most 8192 of its entries are read.  The labels do not
+   depend on the choice. */
+int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, eclcc_params* p);
+
 /* ---- plain drop-in ----------------------------------------------------- */
 
 /* Replaces computeCC (ECL-CC-atomic.cu:275-325): host CSR in, host labels
@@ -93,7 +104,8 @@ int eclcc_compute(int32_t nodes, int32_t edges, const int32_t* nindex, const int
                   int32_t* nstat);
 
 /* Same, and *seconds = device time of the kernel sequence only (the window
-   of the reference's GPUTimer, ECL-CC-atomic.cu:305-312). params may be NULL. */
+   of the reference's GPUTimer, ECL-CC-atomic.cu:305-312).  params NULL:
+   eclcc_params_for_graph's choice (eclcc_compute passes NULL). */
 int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist,
                         int32_t* nstat, const eclcc_params* params, double* seconds);
 
@@ -206,7 +218,11 @@ typedef struct eclcc_server_config {
   int32_t max_inflight;    /* requests resident on the device at once (0 = 4096) */
   int32_t streams;         /* concurrent batch streams (0 = 4) */
   int32_t max_batch;       /* most requests fused into one launch (0 = 1024) */
-  int32_t reserved[5];
+  int32_t batch_wait_us;   /* while another batch is on the GPU, a batch smaller than
+                              max_batch waits until its oldest request has waited
+                              this long (0 = launch whatever is waiting at once); an
+                              idle GPU always takes the waiting requests at once */
+  int32_t reserved[4];
 } eclcc_server_config;
 
 /* Starts the serving thread on the current device. params may be NULL. */

@@ -98,3 +98,26 @@ def test_label_accesses_go_through_atomic_helpers():
     # the one plain store is init's, where no other thread reads labels
     assert plain == ["G.nstat[v] = m;  // no other thread reads labels during init"] or \
         plain == ["G.nstat[v] = m;"], plain
+
+
+def test_params_for_graph_picks_by_degree_skew():
+    # even degrees (grid, Erdos-Renyi): half of the sampled hooks; a heavy
+    # tail (R-MAT): all of them; the rest of the defaults unchanged
+    from eclcc import params_for_graph, workloads
+
+    d = eclcc.default_params().as_dict()
+    for g, shift in ((workloads.grid3d(32, 0.6, 1), 1), (workloads.er65536(1), 1),
+                     (workloads.rmat(16, 16, 2), 0)):
+        p = params_for_graph(g).as_dict()
+        assert p == dict(d, sample_shift=shift), g.name
+    # a star is skewed; a single vertex and an empty graph are not
+    import graphs as G
+
+    assert params_for_graph(G.star(5000, 17)).sample_shift == 0
+    assert params_for_graph(G.empty(1)).sample_shift == 1
+    assert params_for_graph(G.empty(100000)).sample_shift == 1
+    assert params_for_graph(workloads.grid3d(16, 0.6, 1), giant_filter=1).giant_filter == 1
+    L = N.cc_lib()
+    p = N.EclccParams()
+    assert L.eclcc_params_for_graph(0, 0, None, C.byref(p)) == -1
+    assert L.eclcc_params_for_graph(1, 0, None, None) == -1

@@ -136,7 +136,8 @@ def test_served_poisson_1024_requests_alone_and_in_the_mix(cuda):
     qps = 40000.0
     arrivals = np.cumsum(np.random.default_rng(2024).exponential(1.0 / qps, R))
     announce(f"Poisson stream of {R} requests at {qps:.0f}/s")
-    with Server(hbm_budget_bytes=16 << 30, streams=4, max_batch=64, params=params) as s:
+    with Server(hbm_budget_bytes=16 << 30, streams=3, max_batch=128, params=params,
+                batch_wait_us=100) as s:
         sub = L.eclcc_server_submit_device
         ids = []
         t0 = time.perf_counter()
