@@ -426,15 +426,9 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
     hipStream_t s;
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } sg{s};
-  if ((rc = check_hip(hipMemcpyAsync(d_idx, nindex, sizeof(int32_t) * ((size_t)nodes + 1),
-                                     hipMemcpyHostToDevice, s),
-                      ECLCC_ERR_COPY, "copying to device failed")))
-    return rc;
-  if (edges > 0 &&
-      (rc = check_hip(hipMemcpyAsync(d_lst, nlist, sizeof(int32_t) * (size_t)edges,
-                                     hipMemcpyHostToDevice, s),
-                      ECLCC_ERR_COPY, "copying to device failed")))
-    return rc;
+  // host CSR in: large arrays through pinned staging (upload_host)
+  if ((rc = upload_host(d_idx, nindex, sizeof(int32_t) * ((size_t)nodes + 1), s))) return rc;
+  if ((rc = upload_host(d_lst, nlist, sizeof(int32_t) * (size_t)edges, s))) return rc;
   float kms[kNumKernels], tot = 0.f;
   rc = eclcc_run_device_profiled(nodes, edges, (const int32_t*)d_idx, (const int32_t*)d_lst,
                                  (int32_t*)d_st, d_w, wbytes, params, s, kms, &tot);
