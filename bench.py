@@ -151,7 +151,7 @@ def plain(args) -> dict:
     hubs, tasks = dg.hub_stats()
     # roofline of the kernel that takes the most time, on its own algorithmic bytes
     kb = roofline.kernel_bytes(g.nindex, g.nlist, dg.params.thread_max_deg, dg.params.wave_max_deg,
-                               dg.params.giant_filter)
+                               dg.params.giant_filter, dg.params.sample_shift)
     dom = max((k for k in KERNELS if kb[k]), key=lambda k: kernel_ms[k])
     dom_gbs = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9
     out = {
@@ -251,8 +251,9 @@ def plain(args) -> dict:
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, args.cpu_budget, labels)
-    log("drop-in call on host arrays (PCIe included)")
-    out["host_array_call"] = host_array_call(g, labels)
+    if not args.no_host_call:
+        log("drop-in call on host arrays (PCIe included)")
+        out["host_array_call"] = host_array_call(g, labels)
     # the batched and served halves of the metric, on the graph mix
     dg = g = labels = None
     torch.cuda.empty_cache()
@@ -523,18 +524,48 @@ def copy_probe(args) -> dict:
             "GBps": 8 * n / (ms * 1e-3) / 1e9}
 
 
+def gather_probe(args) -> dict:
+    """A random 4-byte gather of known size: out[i] = table[idx[i]] over a
+    64 MiB table (the label array of a 2^24-vertex graph), 2^26 random
+    indices.  The reference point that scales the HBM counters (FETCH_SIZE,
+    the TCC read-request counters) for random gathers, as --mode copy does
+    for streaming: 256 MiB of indices read and 256 MiB written are
+    streaming, the rest of the fetched bytes are the gathers' lines."""
+    import torch
+
+    n_table, n_idx = 1 << 24, 1 << 26
+    g = torch.Generator(device="cuda:0").manual_seed(7)
+    table = torch.arange(n_table, dtype=torch.int32, device="cuda:0")
+    idx = torch.randint(0, n_table, (n_idx,), dtype=torch.int32, device="cuda:0", generator=g)
+    out = torch.empty(n_idx, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        torch.index_select(table, 0, idx, out=out)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    ok = bool(torch.equal(out, idx))
+    return {"mode": "gather", "table_bytes": 4 * n_table, "gathers": n_idx,
+            "stream_bytes_read": 4 * n_idx, "stream_bytes_written": 4 * n_idx,
+            "ms_per_gather_pass": ms, "gathers_per_s": n_idx / (ms * 1e-3), "values_correct": ok}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="plain",
-                    choices=["plain", "batched", "served", "copy", "racy", "mst"])
+                    choices=["plain", "batched", "served", "copy", "gather", "racy", "mst"])
     ap.add_argument("--workload", default="rmat24")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the read-every-edge (giant_filter 1) comparison pass")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-host-call", action="store_true",
+                    help="plain mode: leave out the drop-in call on host arrays")
     ap.add_argument("--no-batched", action="store_true",
                     help="plain mode: leave out the batched object of the line")
     ap.add_argument("--no-served", action="store_true",
@@ -564,6 +595,8 @@ def main() -> None:
         out = plain(args)
     elif args.mode == "copy":
         out = copy_probe(args)
+    elif args.mode == "gather":
+        out = gather_probe(args)
     elif args.mode == "racy":
         out = racy(args)
     elif args.mode == "mst":

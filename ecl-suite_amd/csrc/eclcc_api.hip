@@ -79,7 +79,7 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   if (p.giant_filter < 0 || p.giant_filter > 2)
     return set_error(ECLCC_ERR_ARG, "giant_filter must be 0, 1 or 2");
   t.filter = p.giant_filter;
-  if (p.path_mode != 0 && p.path_mode != 1) return set_error(ECLCC_ERR_ARG, "path_mode must be 0 or 1");
+  if (p.path_mode < 0 || p.path_mode > 2) return set_error(ECLCC_ERR_ARG, "path_mode must be 0, 1 or 2");
   t.path_mode = p.path_mode;
   if (p.sample_shift < 0 || p.sample_shift > 4) return set_error(ECLCC_ERR_ARG, "sample_shift must be in [0, 4]");
   t.sample_shift = p.sample_shift;
@@ -289,7 +289,15 @@ int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, 
   // leaves outside the largest tree have short lists, which compute1 takes
   // cheaply, so half of the sampled hooks suffice; with a heavy tail they
   // include long lists, and every sampled hook pays for itself
-  if ((double)dmax <= 4.0 * avg + 8.0) p->sample_shift = 1;
+  if ((double)dmax <= 4.0 * avg + 8.0) {
+    p->sample_shift = 1;
+  } else {
+    // skewed: the meeting walks climb long shared chains below the hubs'
+    // trees, which halving shortens (R-MAT-24: 0.500 -> 0.491 ms); on even
+    // degrees the walks share the few top labels, and the halving stores
+    // to them contend (ER-65536: 0.060 -> 0.091 ms)
+    p->path_mode = 2;
+  }
   return ECLCC_OK;
 }
 

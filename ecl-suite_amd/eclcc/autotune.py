@@ -28,7 +28,7 @@ SPACE: Dict[str, List[int]] = {
     "hub_chunk": [2048, 4096, 8192],
     "block_threads": [256, 512],
     "giant_filter": [2, 1, 0],
-    "path_mode": [1, 0],
+    "path_mode": [1, 2, 0],
 }
 
 TABLE = N.PKG_ROOT / "tuning" / "mi355x.json"
@@ -107,10 +107,62 @@ def load(key: str, path: Path = TABLE) -> Optional[N.EclccParams]:
     return None if entry is None else _params(entry["params"])
 
 
+# One graph: each parameter varied alone around the per-class choice
+# (eclcc_params_for_graph), and the pairs that trade sampled hooks against
+# the work left to compute1/compute2.
+PLAIN_SPACE: List[Dict[str, int]] = (
+    [dict()]
+    + [dict(sample_shift=s) for s in (0, 1, 2, 3)]
+    + [dict(path_mode=m) for m in (0, 1, 2)]
+    + [dict(thread_max_deg=t) for t in (4, 8, 32, 64)]
+    + [dict(wave_max_deg=w) for w in (256, 512, 1024, 4096, 8192)]
+    + [dict(thread_max_deg=t, sample_shift=s) for t, s in ((8, 1), (32, 1), (64, 1), (8, 2))]
+    + [dict(giant_filter=f) for f in (1, 0)]
+)
+
+
+def sweep_plain(g, space: Optional[List[Dict[str, int]]] = None, reps: int = 20):
+    """Times one graph resident in HBM under each parameter set of `space`
+    (overrides of eclcc_params_for_graph's choice): yields one dict per set
+    with the pass time (HIP events over `reps` back-to-back passes), the
+    per-kernel times (event pairs, mean of 5) and whether the labels equal
+    the first set's."""
+    import numpy as np
+    import torch
+
+    from .cc import KERNELS, DeviceGraph, params_for_graph
+
+    stream = torch.cuda.current_stream()
+    dg = DeviceGraph(g, "cuda:0")
+    ref = None
+    for kw in space or PLAIN_SPACE:
+        dg.set_params(params_for_graph(g, **kw))
+        for _ in range(3):
+            dg.run(stream)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            dg.run(stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        kms = [statistics.mean(x) for x in zip(*[dg.run_profiled(stream)[0] for _ in range(5)])]
+        lab = dg.labels()
+        if ref is None:
+            ref = lab
+        yield {"graph": g.name, "overrides": kw, "params": dg.params.as_dict(),
+               "ms": a.elapsed_time(b) / reps,
+               "kernels_ms": {k: v for k, v in zip(KERNELS, kms)},
+               "labels_equal": bool(np.array_equal(lab, ref))}
+
+
 def main(argv=None) -> None:
     """Tunes on a sample of the batch mix and stores the result in the table
     under "batch_mix" (read by the served mode):
-    python -m eclcc.autotune [--graphs 256] [--budget 120] [--out path]."""
+    python -m eclcc.autotune [--graphs 256] [--budget 120] [--out path].
+    With --plain WORKLOAD (rmat24, grid256, er65536) it sweeps the plain
+    path's parameters on that graph instead and prints one JSON line per
+    parameter set."""
     import argparse
 
     from . import workloads
@@ -119,10 +171,17 @@ def main(argv=None) -> None:
     ap.add_argument("--graphs", type=int, default=256)
     ap.add_argument("--budget", type=float, default=120.0)
     ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--plain", default=None, choices=["rmat24", "grid256", "er65536"])
     a = ap.parse_args(argv)
     import torch
 
     torch.cuda.init()  # before the library touches the device (time_batch times on torch's stream)
+    if a.plain:
+        g = {"rmat24": workloads.rmat24, "grid256": workloads.grid256,
+             "er65536": lambda: workloads.er65536(1)}[a.plain]()
+        for row in sweep_plain(g):
+            print(json.dumps(row), flush=True)
+        return
     graphs = workloads.batch_mix(a.graphs)
     with Batch(graphs) as b:
         best, trials = tune(b, budget_s=a.budget)

@@ -6,9 +6,10 @@ once):  B(n, m) = 4(n+1) + 4m + 4n.
 Per kernel, the bytes its own job cannot avoid moving:
 
     init      4(n+1) + 4n      read the index, write every label
-    sample    4 per sampled vertex (the entry it hooks)
+    sample    16 per sampled vertex: read the entry it hooks and the labels
+              of both ends, write its own label
     hubscan, pick              bookkeeping (no per-vertex or per-edge term)
-    mark      n/8              one bit per vertex
+    mark      4n + n/8         read every label, write one bit per vertex
     compute1  4 per adjacency entry + 8 per vertex of the low-degree bin
     compute2  4 per adjacency entry + 8 per vertex of the medium-degree bin
     compute3  4 per adjacency entry + 8 per vertex of the high-degree bin
@@ -16,8 +17,11 @@ Per kernel, the bytes its own job cannot avoid moving:
 
 A vertex is active when it has a smaller neighbour (only those hook,
 ECL-CC-atomic.cu:119-132).  The sample kernel hooks one entry per vertex:
-the second entry of an active vertex if that is smaller, the first entry
-of an inactive vertex with a neighbour.  The bins split the active
+the second entry of an active vertex if that is smaller (with sample_shift
+s > 0 only for the active vertices whose multiplicative hash
+(v * 0x9E3779B1 mod 2^32) has its top s bits clear), the first entry of an
+inactive vertex with a neighbour.  Labels are counted as read by the
+kernels that need them (sample, mark, flatten), not only as written.  The bins split the active
 vertices by degree:
 low <= thread_max_deg < medium <= wave_max_deg < high.  That is the work of
 giant_filter 0 and 1.  With giant_filter 2 the bins hold only the vertices
@@ -32,8 +36,8 @@ import numpy as np
 
 PER_UNIT = {
     "init": "4(n+1) + 4n bytes per graph",
-    "sample": "4 bytes per sampled vertex",
-    "mark": "1/8 byte per vertex",
+    "sample": "16 bytes per sampled vertex (hooked entry and both labels read, its label written)",
+    "mark": "4 + 1/8 bytes per vertex (label read, membership bit written)",
     "compute1": "4 bytes per entry + 8 bytes per vertex of the low-degree bin",
     "compute2": "4 bytes per entry + 8 bytes per vertex of the medium-degree bin",
     "compute3": "4 bytes per entry + 8 bytes per vertex of the high-degree bin",
@@ -45,7 +49,8 @@ def pass_bytes(n: int, m: int) -> int:
     return 4 * (n + 1) + 4 * m + 4 * n
 
 
-def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) -> Dict[str, tuple]:
+def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
+         sample_shift: int = 0) -> Dict[str, tuple]:
     """(vertices, entries) of the active vertices in each degree bin."""
     n = nindex.shape[0] - 1
     deg = np.diff(nindex).astype(np.int64)
@@ -61,7 +66,11 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) 
     first[has] = nlist[nindex[:-1][has]]
     two = deg >= 2
     second[two] = nlist[nindex[:-1][two] + 1]
-    sampled = np.where(active, second < ids, has & (first != ids))
+    pick = np.ones(n, dtype=bool)
+    if sample_shift:
+        h = (ids.astype(np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
+        pick = (h >> np.uint64(32 - sample_shift)) == 0
+    sampled = np.where(active, (second < ids) & pick, has & (first != ids))
     low = active & (deg <= thread_max)
     mid = active & (deg > thread_max) & (deg <= wave_max)
     hub = active & (deg > wave_max)
@@ -75,18 +84,18 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int) 
 
 
 def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
-                 giant_filter: int = 1) -> Dict[str, Optional[int]]:
+                 giant_filter: int = 1, sample_shift: int = 0) -> Dict[str, Optional[int]]:
     n = nindex.shape[0] - 1
-    b = bins(nindex, nlist, thread_max, wave_max)
+    b = bins(nindex, nlist, thread_max, wave_max, sample_shift)
     per_bin: Dict[str, Optional[int]] = {k: 4 * e + 8 * v for k, (v, e) in b.items()}
     if giant_filter == 2:
         per_bin.update(low=None, mid=None, hub=None)
     return {
         "init": 4 * (n + 1) + 4 * n,
-        "sample": 4 * b["sampled"][0],
+        "sample": 16 * b["sampled"][0],
         "hubscan": 0,
         "pick": 0,
-        "mark": (n + 7) // 8,
+        "mark": 4 * n + (n + 7) // 8,
         "compute1": per_bin["low"],
         "compute2": per_bin["mid"],
         "compute3": per_bin["hub"],

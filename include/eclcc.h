@@ -71,9 +71,11 @@ typedef struct eclcc_params {
   int32_t path_mode;      /* shortening of find paths: 0 = re-point every visited vertex
                              to its grandparent (the reference's intermediate pointer
                              jumping, ECL-CC-atomic.cu:92-104); 1 = re-point only the
-                             start vertex, to the root (one store per walk).  The
-                             sampled hooks walk both paths together to where they meet
-                             in either mode */
+                             start vertex, to the root (one store per walk); 2 = as 1,
+                             and the sampled hooks' meeting walks also halve the path
+                             they climb (a store per two levels).  The sampled hooks
+                             walk both paths together to where they meet in every
+                             mode */
   int32_t sample_shift;   /* the sampled hook of a vertex that has a smaller neighbour
                              (its second entry) is taken by 1 in 2^sample_shift of
                              them, picked by a hash of the ID; 0 (default): by all.
@@ -88,11 +90,12 @@ void eclcc_params_default(eclcc_params* p);
    host from its degree statistics: graphs whose lists all have about the
    same length (3-D grids, Erdos-Renyi: the largest of 4096 evenly spaced
    degrees is at most 4 x the average + 8) take the sampled second-entry
-   hooks of half of their vertices (sample_shift 1; grid 256^3: 1.09 -> 0.97
+   hooks of half of their vertices (sample_shift 1; grid 256^3: 1.09 -> 0.96
    ms per pass on MI355X); skewed degree distributions (R-MAT, power-law)
-   keep them all (sample_shift 0; R-MAT-24: 0.50 ms, 0.59 with 1).  nindex
-   is host memory; at most 8192 of its entries are read.  The labels do not
-   depend on the choice. */
+   keep them all and halve the paths their meeting walks climb
+   (sample_shift 0, path_mode 2; R-MAT-24: 0.500 -> 0.491 ms, 0.59 with
+   sample_shift 1).  nindex is host memory; at most 8192 of its entries are
+   read.  The labels do not depend on the choice. */
 int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, eclcc_params* p);
 
 /* ---- plain drop-in ----------------------------------------------------- */

@@ -43,7 +43,7 @@ def test_params_defaults_and_workspace():
     for sh in range(5):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(sample_shift=sh))) > 0
     for bad in (eclcc.default_params(giant_filter=3), eclcc.default_params(giant_filter=-1),
-                eclcc.default_params(path_mode=2), eclcc.default_params(sample_shift=5),
+                eclcc.default_params(path_mode=3), eclcc.default_params(sample_shift=5),
                 eclcc.default_params(sample_shift=-1)):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
 
@@ -106,10 +106,10 @@ def test_params_for_graph_picks_by_degree_skew():
     from eclcc import params_for_graph, workloads
 
     d = eclcc.default_params().as_dict()
-    for g, shift in ((workloads.grid3d(32, 0.6, 1), 1), (workloads.er65536(1), 1),
-                     (workloads.rmat(16, 16, 2), 0)):
+    for g, shift, mode in ((workloads.grid3d(32, 0.6, 1), 1, 1), (workloads.er65536(1), 1, 1),
+                           (workloads.rmat(16, 16, 2), 0, 2)):
         p = params_for_graph(g).as_dict()
-        assert p == dict(d, sample_shift=shift), g.name
+        assert p == dict(d, sample_shift=shift, path_mode=mode), g.name
     # a star is skewed; a single vertex and an empty graph are not
     import graphs as G
 

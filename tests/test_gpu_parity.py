@@ -76,6 +76,9 @@ PARAMS = [
     dict(sample_shift=4),
     dict(sample_shift=2, giant_filter=1),
     dict(sample_shift=3, giant_filter=0, path_mode=0),
+    dict(path_mode=2),
+    dict(path_mode=2, giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=128),
+    dict(path_mode=2, giant_filter=1, sample_shift=2),
 ]
 
 
@@ -164,6 +167,11 @@ def test_full_size_grid256(cuda):
     dg2 = DeviceGraph(g, "cuda:0", default_params(giant_filter=other))
     dg2.run()
     assert np.array_equal(dg2.labels(), lab)
+    for kw in (dict(path_mode=2), dict(path_mode=2, sample_shift=1)):
+        announce(f"full size {g.name} {kw}")
+        dg2.set_params(default_params(**kw))
+        dg2.run()
+        assert np.array_equal(dg2.labels(), lab), kw
     del dg2
     # size-independent properties
     assert np.all(lab <= np.arange(g.nodes)) and np.all(lab[lab] == lab)
@@ -172,9 +180,9 @@ def test_full_size_grid256(cuda):
 def test_full_size_rmat24(cuda):
     g = workloads.rmat24()
     first = None
-    for filt in (1, 2):
-        announce(f"full size {g.name} n={g.nodes} m={g.edges} giant_filter={filt}")
-        dg = DeviceGraph(g, "cuda:0", default_params(giant_filter=filt))
+    for filt, mode in ((1, 1), (2, 1), (2, 2)):
+        announce(f"full size {g.name} n={g.nodes} m={g.edges} giant_filter={filt} path_mode={mode}")
+        dg = DeviceGraph(g, "cuda:0", default_params(giant_filter=filt, path_mode=mode))
         dg.run()
         lab = dg.labels()
         hubs, tasks = dg.hub_stats()
@@ -196,11 +204,12 @@ def test_filter_and_path_modes_agree_full_rmat20(cuda):
     ref = compute(g, default_params(giant_filter=1))
     check(g, ref)
     for kw in (dict(giant_filter=0), dict(path_mode=0), dict(path_mode=0, giant_filter=0),
-               dict(giant_filter=2), dict(giant_filter=2, path_mode=0)):
+               dict(giant_filter=2), dict(giant_filter=2, path_mode=0), dict(path_mode=2),
+               dict(path_mode=2, giant_filter=0), dict(path_mode=2, sample_shift=1)):
         assert np.array_equal(compute(g, default_params(**kw)), ref), kw
 
 
-@pytest.mark.parametrize("mode,filt", [(0, 1), (1, 1), (0, 2), (1, 2), (1, 0)])
+@pytest.mark.parametrize("mode,filt", [(0, 1), (1, 1), (0, 2), (1, 2), (1, 0), (2, 2), (2, 1), (2, 0)])
 def test_path_and_filter_modes_on_grid_and_er(cuda, mode, filt):
     for g in (workloads.grid3d(48, 0.6, 4), workloads.er65536(2), workloads.grid3d(40, 0.25, 5)):
         announce(f"path_mode={mode} giant_filter={filt} {g.name}")
