@@ -46,6 +46,13 @@ def _labels_equal(a: List[np.ndarray], b: List[np.ndarray]) -> bool:
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+def BASELINE_PARAMS():
+    """Parameters of the one-graph-per-launch baselines: the built-in
+    defaults, the fastest of the measured sets for the mix's graphs run one
+    at a time."""
+    return default_params()
+
+
 def batch_params(args, graphs):
     """The batch's parameters: the committed autotuner table (tuning/
     mi355x.json, key batch_mix) unless a tuning budget is given, in which
@@ -102,8 +109,12 @@ def batched(args, check=None) -> dict:
         log("labels against the oracle (outside the timed region)")
         oracle_ok = bool(check(graphs, lab_batch))
 
-    # one graph per launch, synchronising after each (the plain API's usage)
+    # one graph per launch, synchronising after each (the plain API's usage),
+    # with the built-in defaults: the fastest parameters for graphs of this
+    # mix run one at a time (13.4 K graphs/s on the served mix, against 9.1 K
+    # with the batch's tuned path_mode 2, which pays off only in full batches)
     log("one graph per launch")
+    b.set_params(BASELINE_PARAMS())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for g in range(count):
@@ -148,6 +159,7 @@ def batched(args, check=None) -> dict:
         "labels_batched_equal_plain": bool(same),
         "labels_equal_oracle": oracle_ok,
         "params_source": params_source,
+        "baseline_params": BASELINE_PARAMS().as_dict(),
         "kernels_ms": kms,
     }
     b.close()
@@ -194,7 +206,9 @@ def served(args, check=None) -> dict:
     n_tot = sum(g.nodes for g in graphs)
     m_tot = sum(g.edges for g in graphs)
     log(f"generated {R} requests ({m_tot} CSR entries) in {time.perf_counter() - t0:.1f}s")
-    params = autotune.load("batch_mix") or default_params()
+    # the server's batches are smaller than a whole mix: its own tuned entry
+    params = autotune.load("served") or autotune.load("batch_mix") or default_params()
+    base = BASELINE_PARAMS()
 
     # the graph store: every request's CSR resident in HBM, labels in place
     ni_all = torch.empty(sum(g.nodes + 1 for g in graphs), dtype=torch.int32, device=dev)
@@ -217,14 +231,14 @@ def served(args, check=None) -> dict:
     L = N.cc_lib()
     max_n = max(g.nodes for g in graphs)
     max_m = max(g.edges for g in graphs)
-    wb = int(L.eclcc_workspace_bytes(max_n, max_m, C.byref(params)))
+    wb = int(L.eclcc_workspace_bytes(max_n, max_m, C.byref(base)))
     d_w = torch.empty(wb, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
 
     def plain_one(i):
         n, m, ni, nl, lab = store[i]
         N.check_cc(L.eclcc_run_device(n, m, ni.data_ptr(), nl.data_ptr(), lab.data_ptr(),
-                                      d_w.data_ptr(), wb, C.byref(params), stream.cuda_stream))
+                                      d_w.data_ptr(), wb, C.byref(base), stream.cuda_stream))
         stream.synchronize()
 
     plain_one(0)
@@ -368,7 +382,7 @@ def served(args, check=None) -> dict:
                    "hbm_budget_bytes": budget,
                    "streams": args.serve_streams, "max_batch": args.serve_max_batch,
                    "batch_wait_us": args.serve_wait_us,
-                   "params": params.as_dict()},
+                   "params": params.as_dict(), "baseline_params": base.as_dict()},
         "p50_ms": med["p50_ms"],
         "p99_ms": med["p99_ms"],
         "mean_ms": med["mean_ms"],

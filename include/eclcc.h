@@ -87,7 +87,9 @@ typedef struct eclcc_params {
 void eclcc_params_default(eclcc_params* p);
 
 /* Fills *p with the defaults tuned for this graph's class, chosen on the
-   host from its degree statistics: graphs whose lists all have about the
+   host from its degree statistics.  Graphs of fewer than 2^20 vertices
+   keep the built-in defaults; for larger ones, graphs whose lists all have
+   about the
    same length (3-D grids, Erdos-Renyi: the largest of 4096 evenly spaced
    degrees is at most 4 x the average + 8) take the sampled second-entry
    hooks of half of their vertices (sample_shift 1; grid 256^3: 1.09 -> 0.96

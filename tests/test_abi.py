@@ -101,21 +101,25 @@ def test_label_accesses_go_through_atomic_helpers():
 
 
 def test_params_for_graph_picks_by_degree_skew():
-    # even degrees (grid, Erdos-Renyi): half of the sampled hooks; a heavy
-    # tail (R-MAT): all of them; the rest of the defaults unchanged
-    from eclcc import params_for_graph, workloads
+    # graphs of 2^20 vertices or more: even degrees (grid, Erdos-Renyi) take
+    # half of the sampled hooks, a heavy tail (R-MAT) the halving walks;
+    # smaller graphs and the rest of the parameters keep the defaults
+    from eclcc import Graph, params_for_graph, workloads
 
     d = eclcc.default_params().as_dict()
-    for g, shift, mode in ((workloads.grid3d(32, 0.6, 1), 1, 1), (workloads.er65536(1), 1, 1),
-                           (workloads.rmat(16, 16, 2), 0, 2)):
+    for g, shift, mode in ((workloads.grid3d(104, 0.6, 1), 1, 1), (Graph.er(1 << 20, 4 << 20, 3), 1, 1),
+                           (workloads.rmat(20, 16, 2), 0, 2), (workloads.rmat(16, 16, 2), 0, 1),
+                           (workloads.er65536(1), 0, 1), (workloads.grid3d(32, 0.6, 1), 0, 1)):
         p = params_for_graph(g).as_dict()
         assert p == dict(d, sample_shift=shift, path_mode=mode), g.name
-    # a star is skewed; a single vertex and an empty graph are not
+    # a large star is skewed; a large empty graph is not; small ones keep the defaults
     import graphs as G
 
-    assert params_for_graph(G.star(5000, 17)).sample_shift == 0
-    assert params_for_graph(G.empty(1)).sample_shift == 1
-    assert params_for_graph(G.empty(100000)).sample_shift == 1
+    # (the hub at a sampled position: the choice reads 4096 evenly spaced degrees)
+    star = Graph.from_edges(1 << 20, np.zeros((1 << 20) - 1, dtype=np.int32), np.arange(1, 1 << 20))
+    assert params_for_graph(star).as_dict() == dict(d, path_mode=2)
+    assert params_for_graph(G.empty(1 << 20)).sample_shift == 1
+    assert params_for_graph(G.empty(1)).as_dict() == d
     assert params_for_graph(workloads.grid3d(16, 0.6, 1), giant_filter=1).giant_filter == 1
     L = N.cc_lib()
     p = N.EclccParams()

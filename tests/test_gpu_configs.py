@@ -51,9 +51,9 @@ def first_mismatch(graphs, got, expect):
     return None
 
 
-def tuned_params():
-    p = autotune.load("batch_mix")
-    assert p is not None, "tuning/mi355x.json has no batch_mix entry"
+def tuned_params(key="batch_mix"):
+    p = autotune.load(key)
+    assert p is not None, f"tuning/mi355x.json has no {key} entry"
     return p
 
 
@@ -105,7 +105,7 @@ def test_served_poisson_1024_requests_alone_and_in_the_mix(cuda):
     announce(f"generated {R} request graphs in {time.perf_counter() - t0:.0f}s, "
              f"m={sum(g.edges for g in graphs)}")
     expect = oracle_labels(graphs)
-    params = tuned_params()
+    params = tuned_params("served")
     # the graph store: every request's CSR resident in HBM
     store = []
     for g in graphs:
