@@ -216,7 +216,7 @@ def plain(args) -> dict:
             gbs = t["bytes_per_pass"] / (pass_ms * 1e-3) / 1e9
             out["pass_roofline"]["traffic_GBps"] = gbs
             out["pass_roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
-            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3: {t.get('source', 'FETCH_SIZE + WRITE_SIZE, copy-calibrated')})"
     # the same graph with every adjacency list read (giant_filter 1)
     if dg.params.giant_filter == 2 and not args.no_compare:
         del dg
@@ -500,7 +500,7 @@ def mst_bench(args) -> dict:
             out["pass_roofline"]["traffic"] = t["bytes_per_forest"]
             out["pass_roofline"]["traffic_GBps"] = gbs
             out["pass_roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
-            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+            out["traffic_source"] = f"profiles/{tr.name} (rocprofv3: {t.get('source', 'FETCH_SIZE + WRITE_SIZE, copy-calibrated')})"
     return out
 
 
