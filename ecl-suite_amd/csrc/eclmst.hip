@@ -29,8 +29,9 @@
 //     nindex, then maps entries to vertices through an LDS window of nindex.
 //   * Between the light and the heavy phase every vertex is pointed at its
 //     root, so the heavy phase reads each end's root with one load.
-//   * A component minimum is reset only if it is not reset already: the
-//     stores of a large tree's edges would all hit its root's entry.
+//   * The component minima are reset by one coalesced pass over the array,
+//     not per listed edge: the stores of a large tree's edges would all hit
+//     its root's entry, and stores to one address serialise.
 //   * Race freedom by construction: parent and minv are only touched through
 //     relaxed agent-scope atomics; each in_mst byte has one writer.
 
@@ -284,6 +285,19 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
   long long t0 = jb0;
   int iw = s_i0;
   while (t0 < jb1) {
+    // The tile's entries and weights do not depend on the nindex window, so
+    // their loads are issued first and their latency overlaps the window's
+    // (R-MAT-24 forest 12.5 -> 11.9 ms with the proposal-only first round).
+    // Entries past the window's last vertex are loaded again by the next
+    // tile.
+    const long long tmax = std::min<long long>(t0 + kTile, jb1);
+    int pk[kPer], pw[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+      const long long j = t0 + threadIdx.x + r * kThreads;
+      pk[r] = j < tmax ? nlist[j] : 0;
+      pw[r] = j < tmax ? eweight[j] : 0;
+    }
     for (int x = threadIdx.x; x <= kWin; x += kThreads) s_idx[x] = nindex[std::min<long long>((long long)iw + x, n)];
     __syncthreads();
     const long long te = std::max<long long>(t0, std::min<long long>(std::min<long long>(t0 + kTile, jb1), s_idx[kWin]));
@@ -298,8 +312,8 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
         it[r] = make_int4(0, 0, 0, 0);
         if (j < te) {
           const int i = iw + window_slot(s_idx, j);
-          const int k = nlist[j];
-          const int w = eweight[j];
+          const int k = pk[r];  // j < te <= tmax
+          const int w = pw[r];
           neg |= w < 0;
           if (k > i && (HEAVY ? w > thr : w <= thr)) {
             if (HEAVY) {
@@ -333,11 +347,16 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
 // vertex, and in later rounds the many edges into the largest tree) first
 // take their minimum across the wavefront and send one atomic for all of
 // them: every atomic on one address executes at the memory side, one after
-// the other.  All lanes of the wavefront call it.
+// the other.  The other lanes propose on their own after it.  (Written with
+// the combined proposal first and the others after it, this runs 1.9x
+// faster on R-MAT-24 than the if/else form the compiler laid out the other
+// way round: 1.91 -> 1.03 ms for the two proposal-only rounds.)  All lanes
+// of the wavefront call it.
 __device__ __forceinline__ void propose(unsigned long long* minv, int root, unsigned long long key,
                                         bool active) {
   const unsigned long long act = __ballot(active);
   if (!act) return;
+  const int lane = lane_id();
   const int first = __ffsll((long long)act) - 1;
   const int r0 = __shfl(root, first);
   const bool same = active && root == r0;
@@ -347,11 +366,9 @@ __device__ __forceinline__ void propose(unsigned long long* minv, int root, unsi
     const unsigned long long o = __shfl_xor(k, d);
     k = o < k ? o : k;
   }
-  if (same) {
-    if (lane_id() == first) min64(&minv[r0], k);
-  } else if (active) {
-    min64(&minv[root], key);
-  }
+  if (lane == first) min64(&minv[r0], k);
+  const unsigned long long rest = act & ~__ballot(same);
+  if ((rest >> lane) & 1ull) min64(&minv[root], key);
 }
 
 // kernel1 (:240-256): keeps the listed edges whose ends are still in
@@ -395,6 +412,29 @@ __global__ __launch_bounds__(kThreads) void k_link(const int4* in, const unsigne
     append(A, it, ok, out, out_cnt, cap, flags);
   }
   flush(A, out, out_cnt, cap, flags);
+}
+
+// The first round of a phase (kernel1 :240-256 on the list initWL wrote):
+// every listed end is already a root -- the light phase starts from single
+// vertices, and the heavy phase lists the roots of a flattened forest with
+// no union since -- and the two ends of every item differ.  kernel1 would
+// find each end's root (two gathers) and write the whole list out again
+// unchanged, so this round only proposes each item to both minima, and
+// kernel2 reads the list initWL wrote (R-MAT-24 forest 12.7 -> 12.5 ms,
+// grid 256^3 10.0 -> 9.2 ms).
+__global__ __launch_bounds__(kThreads) void k_propose(const int4* in, const unsigned* in_cnt,
+                                                      unsigned cap, unsigned long long* minv) {
+  const long long total = std::min<long long>(*in_cnt, cap);
+  const long long stride = (long long)gridDim.x * kThreads;
+  // block-uniform loop: propose() takes all lanes of the wavefront
+  for (long long base = (long long)blockIdx.x * kThreads; base < total; base += stride) {
+    const long long i = base + threadIdx.x;
+    const bool ok = i < total;
+    const int4 e = ok ? in[i] : make_int4(0, 0, 0, 0);
+    const unsigned long long key = key_of(e);
+    propose(minv, e.x, key, ok);
+    propose(minv, e.y, key, ok);
+  }
 }
 
 // kernel2 (:259-271): an edge that is the minimum of either of its trees
@@ -594,14 +634,23 @@ struct Runner {
     return rc;
   }
 
-  // Boruvka rounds on list `a` (its size already in ctrl->count[a]).
-  // After the first round's unions the forest is flattened once: that round
-  // joins the most trees, and the CAS links it leaves form chains that the
-  // second round, still on nearly the whole list, would otherwise walk for
-  // every edge (R-MAT-24: 13.8 -> 12.85 ms per forest; flattening after
-  // every round costs more than it saves).
+  // Boruvka rounds on list `a` (its size already in ctrl->count[a]).  The
+  // first round needs no finds and no new list (k_propose), and is queued
+  // without reading a size back.  After its unions the forest is flattened
+  // once: that round joins the most trees, and the CAS links it leaves form
+  // chains that the second round, still on nearly the whole list, would
+  // otherwise walk for every edge (R-MAT-24: 13.8 -> 12.85 ms per forest;
+  // flattening after every round costs more than it saves).
   int rounds(int a, unsigned cnt) {
-    bool first = true;
+    if (cnt > 0) {  // first round: proposals straight from list a (k_propose)
+      const int g = (int)std::min<long long>((cnt + kThreads - 1) / kThreads, (long long)blocks * 8);
+      k_propose<<<g, kThreads, 0, s>>>(L.wl[a], &L.ctrl->count[a], L.cap, L.minv);
+      k_join<<<g, kThreads, 0, s>>>(L.wl[a], &L.ctrl->count[a], L.cap, L.parent, L.minv, in_mst);
+      const int gr = (int)std::min<long long>((n / 2 + kThreads - 1) / kThreads, (long long)blocks * 8);
+      k_reset<<<std::max(gr, 1), kThreads, 0, s>>>(n, L.minv);
+      const int gf = (int)std::min<long long>(((long long)n + kThreads - 1) / kThreads, (long long)blocks * 8);
+      k_flatten<<<std::max(gf, 1), kThreads, 0, s>>>(n, L.parent);
+    }
     while (cnt > 0) {
       const int b = 1 - a;
       k_clear_count<<<1, 1, 0, s>>>(&L.ctrl->count[b]);
@@ -621,11 +670,6 @@ struct Runner {
         k_join<<<g, kThreads, 0, s>>>(L.wl[b], &L.ctrl->count[b], L.cap, L.parent, L.minv, in_mst);
         const int gr = (int)std::min<long long>((n / 2 + kThreads - 1) / kThreads, (long long)blocks * 8);
         k_reset<<<std::max(gr, 1), kThreads, 0, s>>>(n, L.minv);
-        if (first) {
-          const int gf = (int)std::min<long long>(((long long)n + kThreads - 1) / kThreads, (long long)blocks * 8);
-          k_flatten<<<std::max(gf, 1), kThreads, 0, s>>>(n, L.parent);
-          first = false;
-        }
       }
       a = b;
     }
