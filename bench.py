@@ -525,31 +525,25 @@ def copy_probe(args) -> dict:
 
 
 def gather_probe(args) -> dict:
-    """A random 4-byte gather of known size: out[i] = table[idx[i]] over a
-    64 MiB table (the label array of a 2^24-vertex graph), 2^26 random
-    indices.  The reference point that scales the HBM counters (FETCH_SIZE,
-    the TCC read-request counters) for random gathers, as --mode copy does
-    for streaming: 256 MiB of indices read and 256 MiB written are
-    streaming, the rest of the fetched bytes are the gathers' lines."""
-    import torch
+    """Random 4-byte gathers of known size, the reference point for the label
+    walks: ecl-suite_amd/bin/gather_probe gathers 2^26 uniformly random
+    entries of a 64 MiB table (the label array of a 2^24-vertex graph), one
+    independent gather per lane, with plain, relaxed agent-scope (the
+    library's label loads) and non-temporal loads, then copies 256 MiB.
+    Under rocprofv3 --pmc TCC_EA0_RDREQ_* it gives each flavour's request
+    sizes; here its JSON lines are gathered into one."""
+    import subprocess
 
-    n_table, n_idx = 1 << 24, 1 << 26
-    g = torch.Generator(device="cuda:0").manual_seed(7)
-    table = torch.arange(n_table, dtype=torch.int32, device="cuda:0")
-    idx = torch.randint(0, n_table, (n_idx,), dtype=torch.int32, device="cuda:0", generator=g)
-    out = torch.empty(n_idx, dtype=torch.int32, device="cuda:0")
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(args.steps):
-        torch.index_select(table, 0, idx, out=out)
-    end.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
-    ok = bool(torch.equal(out, idx))
-    return {"mode": "gather", "table_bytes": 4 * n_table, "gathers": n_idx,
-            "stream_bytes_read": 4 * n_idx, "stream_bytes_written": 4 * n_idx,
-            "ms_per_gather_pass": ms, "gathers_per_s": n_idx / (ms * 1e-3), "values_correct": ok}
+    exe = REPO / "ecl-suite_amd" / "bin" / "gather_probe"
+    if not exe.exists():
+        raise SystemExit(f"{exe} is not built (run __graft_entry__.build())")
+    r = subprocess.run([str(exe), str(max(2, min(args.steps, 1000)))], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        raise SystemExit(f"gather_probe failed ({r.returncode}): {r.stderr.strip()}")
+    rows = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    return {"mode": "gather", "kernels": {row.pop("kernel"): row for row in rows},
+            "values_correct": all(row["values_correct"] for row in rows)}
 
 
 def main() -> None:

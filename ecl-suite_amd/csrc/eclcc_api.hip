@@ -295,6 +295,12 @@ int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, 
   if (nodes < (1 << 20)) return ECLCC_OK;
   if ((double)dmax <= 4.0 * avg + 8.0) {
     p->sample_shift = 1;
+    // with 2^23 or more vertices the label array (32 MB and up) lives
+    // outside L2 and each walk step is a far fetch, so halving pays for its
+    // stores (ER n=2^24 degree 6: 1.110 -> 0.896 ms; grid 256^3: 0.955 ->
+    // 0.946 ms); on smaller ones it does not (grid 128^3: 0.178 -> 0.226 ms,
+    // grid 160^3: 0.290 -> 0.317 ms)
+    if (nodes >= (1 << 23)) p->path_mode = 2;
   } else {
     // skewed: the meeting walks climb long shared chains below the hubs'
     // trees, which halving shortens (R-MAT-24: 0.500 -> 0.491 ms); where

@@ -93,7 +93,9 @@ void eclcc_params_default(eclcc_params* p);
    same length (3-D grids, Erdos-Renyi: the largest of 4096 evenly spaced
    degrees is at most 4 x the average + 8) take the sampled second-entry
    hooks of half of their vertices (sample_shift 1; grid 256^3: 1.09 -> 0.96
-   ms per pass on MI355X); skewed degree distributions (R-MAT, power-law)
+   ms per pass on MI355X), and from 2^23 vertices on also halve the paths
+   of their meeting walks (path_mode 2; Erdos-Renyi n = 2^24, degree 6:
+   1.11 -> 0.90 ms); skewed degree distributions (R-MAT, power-law)
    keep them all and halve the paths their meeting walks climb
    (sample_shift 0, path_mode 2; R-MAT-24: 0.500 -> 0.491 ms, 0.59 with
    sample_shift 1).  nindex is host memory; at most 8192 of its entries are

@@ -102,8 +102,9 @@ def test_label_accesses_go_through_atomic_helpers():
 
 def test_params_for_graph_picks_by_degree_skew():
     # graphs of 2^20 vertices or more: even degrees (grid, Erdos-Renyi) take
-    # half of the sampled hooks, a heavy tail (R-MAT) the halving walks;
-    # smaller graphs and the rest of the parameters keep the defaults
+    # half of the sampled hooks, and from 2^23 vertices the halving walks
+    # too; a heavy tail (R-MAT) takes the halving walks; smaller graphs and
+    # the rest of the parameters keep the defaults
     from eclcc import Graph, params_for_graph, workloads
 
     d = eclcc.default_params().as_dict()
@@ -118,7 +119,9 @@ def test_params_for_graph_picks_by_degree_skew():
     # (the hub at a sampled position: the choice reads 4096 evenly spaced degrees)
     star = Graph.from_edges(1 << 20, np.zeros((1 << 20) - 1, dtype=np.int32), np.arange(1, 1 << 20))
     assert params_for_graph(star).as_dict() == dict(d, path_mode=2)
-    assert params_for_graph(G.empty(1 << 20)).sample_shift == 1
+    assert params_for_graph(G.empty(1 << 20)).as_dict() == dict(d, sample_shift=1)
+    assert params_for_graph(G.empty((1 << 23) - 1)).as_dict() == dict(d, sample_shift=1)
+    assert params_for_graph(G.empty(1 << 23)).as_dict() == dict(d, sample_shift=1, path_mode=2)
     assert params_for_graph(G.empty(1)).as_dict() == d
     assert params_for_graph(workloads.grid3d(16, 0.6, 1), giant_filter=1).giant_filter == 1
     L = N.cc_lib()
