@@ -53,7 +53,7 @@ constexpr int kThreads = 256;
 constexpr int kPer = 4;                      // items per thread per step
 constexpr int kTile = kThreads * kPer;       // items per block step
 constexpr int kBuf = kTile;                  // LDS output buffer (int4 entries)
-constexpr int kWin = 256;                    // vertices in the LDS nindex window
+constexpr int kWin = kThreads - 1;           // vertices in the LDS nindex window (one entry per thread)
 constexpr int kSamples = 256;                // weights sampled for the threshold
 constexpr unsigned long long kNoEdge = ~0ull;
 
@@ -190,6 +190,65 @@ __device__ void append(Appender& A, const int4 (&it)[kPer], const bool (&ok)[kPe
   __syncthreads();
 }
 
+// The same buffer with its fill count kept in registers by every thread
+// (each computes the same block totals), so an append costs one block
+// barrier instead of three.  k_first uses it (with the step mapping below,
+// R-MAT-24: 1.89 -> 1.31 ms for the light list); k_link keeps the other
+// (with this one its rounds took 4.25 instead of 3.49 ms per forest).
+struct RegAppender {
+  int4 buf[kBuf];
+  unsigned base;
+  int wsum[2][kThreads / 64];  // per-wavefront counts, double-buffered by step parity
+};
+
+// Writes the buffer's `fill` items (block-uniform) to the list.  All threads
+// of the block call it; its first barrier also orders every thread's buffer
+// writes before the reads here, its second the reads before later writes.
+__device__ void flush_reg(RegAppender& A, int fill, int4* wl, unsigned* cnt, unsigned cap, unsigned* flags) {
+  if (fill == 0) return;  // block-uniform
+  if (threadIdx.x == 0)
+    A.base = __hip_atomic_fetch_add(cnt, (unsigned)fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const unsigned base = A.base;
+  for (int i = threadIdx.x; i < fill; i += kThreads) {
+    const unsigned long long p = (unsigned long long)base + (unsigned)i;
+    if (p < cap) wl[p] = A.buf[i];
+    else __hip_atomic_fetch_or(flags, kFlagOverflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
+// Appends this thread's items with ok set; all threads of the block call it
+// with the same `fill` (the buffer's item count, kept in registers by every
+// thread) and `par` (the step's parity: the counts of consecutive steps use
+// different slots, so no barrier is needed after reading them).
+__device__ void append_reg(RegAppender& A, int& fill, int par, const int4 (&it)[kPer], const bool (&ok)[kPer],
+                       int4* wl, unsigned* cnt, unsigned cap, unsigned* flags) {
+  int c = 0;
+#pragma unroll
+  for (int r = 0; r < kPer; r++) c += ok[r];
+  const int incl = wave_incl_sum(c);
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 63) A.wsum[par][wv] = incl;
+  __syncthreads();
+  int off = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; k++) {
+    const int x = A.wsum[par][k];
+    off += k < wv ? x : 0;
+    total += x;
+  }
+  if (fill + total > kBuf) {  // block-uniform
+    flush_reg(A, fill, wl, cnt, cap, flags);
+    fill = 0;
+  }
+  int pos = fill + off + incl - c;
+#pragma unroll
+  for (int r = 0; r < kPer; r++)
+    if (ok[r]) A.buf[pos++] = it[r];
+  fill += total;
+}
+
 // ---- kernels -------------------------------------------------------------------
 
 // initPM (:177-184) and the zeroed output.
@@ -246,20 +305,29 @@ __device__ __forceinline__ int window_slot(const int* s, long long key) {
 // k > i and weight <= threshold, as (i, k, w, j).  HEAVY = true (after
 // k_flatten): those with weight > threshold whose ends lie in different
 // trees, as (root i, root k, w, j).  Every weight is checked for sign.
+// A step takes kTile entries: each thread four consecutive ones, loaded as
+// 16-byte vectors when both arrays are 16-byte aligned (`vec`), and finds
+// the vertex of its first entry by one binary search in the LDS window of
+// nindex and of the next three by stepping forward.  (One search per entry,
+// with the entries of a thread kTile / 4 apart, made the searches' LDS
+// reads the step's bottleneck: a pass that only streams the same bytes
+// took 0.78 ms where the list build took 1.87.)  The window of the step
+// after is loaded into the other half of s_idx, so no barrier separates
+// this step's reads of it from the next step's writes.
 template <bool HEAVY>
 __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const int* nindex,
                                                     const int* nlist, const int* eweight,
                                                     int* parent, const unsigned long long* gbits,
                                                     Ctrl* ctrl, int4* wl, unsigned cap,
-                                                    long long per) {
-  __shared__ Appender A;
-  __shared__ int s_idx[kWin + 1];
+                                                    long long per, bool vec) {
+  static_assert(kWin + 1 == kThreads && kTile == 4 * kThreads && kPer == 4, "one window entry and four list entries per thread");
+  __shared__ RegAppender A;
+  __shared__ int s_idx[2][kWin + 1];
   __shared__ int s_i0;
   const int thr = ctrl->threshold;
   const bool giant = HEAVY && ctrl->giant >= 0;
   const long long jb0 = (long long)blockIdx.x * per;
   const long long jb1 = std::min<long long>(m, jb0 + per);
-  if (threadIdx.x == 0) A.fill = 0;
   if (jb0 >= jb1) return;  // block-uniform
   // vertex of entry jb0: the last i with nindex[i] <= jb0, by a 64-way search
   if (threadIdx.x < 64) {
@@ -284,62 +352,75 @@ __global__ __launch_bounds__(kThreads) void k_first(int n, long long m, const in
   unsigned* const flags = &ctrl->flags;
   long long t0 = jb0;
   int iw = s_i0;
+  int fill = 0;
+  int par = 0;
   while (t0 < jb1) {
-    // The tile's entries and weights do not depend on the nindex window, so
-    // their loads are issued first and their latency overlaps the window's
-    // (R-MAT-24 forest 12.5 -> 11.9 ms with the proposal-only first round).
-    // Entries past the window's last vertex are loaded again by the next
-    // tile.
-    const long long tmax = std::min<long long>(t0 + kTile, jb1);
+    // four consecutive entries per thread from a 16-byte aligned base; the
+    // entries below t0 (at most three) are not this step's.  Their loads do
+    // not depend on the window and are issued before it.
+    const long long ta = t0 & ~3ll;
+    const long long tmax = std::min<long long>(ta + kTile, jb1);
+    const long long j0 = ta + 4 * (long long)threadIdx.x;
     int pk[kPer], pw[kPer];
+    if (vec && j0 + 3 < tmax) {
+      const int4 k4 = *reinterpret_cast<const int4*>(nlist + j0);
+      const int4 w4 = *reinterpret_cast<const int4*>(eweight + j0);
+      pk[0] = k4.x; pk[1] = k4.y; pk[2] = k4.z; pk[3] = k4.w;
+      pw[0] = w4.x; pw[1] = w4.y; pw[2] = w4.z; pw[3] = w4.w;
+    } else {
 #pragma unroll
-    for (int r = 0; r < kPer; r++) {
-      const long long j = t0 + threadIdx.x + r * kThreads;
-      pk[r] = j < tmax ? nlist[j] : 0;
-      pw[r] = j < tmax ? eweight[j] : 0;
+      for (int q = 0; q < kPer; q++) {
+        pk[q] = j0 + q < tmax ? nlist[j0 + q] : 0;
+        pw[q] = j0 + q < tmax ? eweight[j0 + q] : 0;
+      }
     }
-    for (int x = threadIdx.x; x <= kWin; x += kThreads) s_idx[x] = nindex[std::min<long long>((long long)iw + x, n)];
+    int* const W = s_idx[par];  // nindex[iw .. iw + kWin], one entry per thread
+    W[threadIdx.x] = nindex[std::min<long long>((long long)iw + threadIdx.x, n)];
     __syncthreads();
-    const long long te = std::max<long long>(t0, std::min<long long>(std::min<long long>(t0 + kTile, jb1), s_idx[kWin]));
-    if (te > t0) {
+    const long long te = std::max<long long>(t0, std::min<long long>(tmax, W[kWin]));
+    if (te > t0) {  // block-uniform
       int4 it[kPer];
       bool ok[kPer];
       bool neg = false;
+      int slot = 0, nxt = 0;
+      if (j0 + 3 >= t0 && j0 < te) {
+        slot = window_slot(W, std::max<long long>(j0, t0));
+        nxt = W[slot + 1];  // slot < kWin: j < te <= W[kWin]
+      }
 #pragma unroll
-      for (int r = 0; r < kPer; r++) {
-        const long long j = t0 + threadIdx.x + r * kThreads;
-        ok[r] = false;
-        it[r] = make_int4(0, 0, 0, 0);
-        if (j < te) {
-          const int i = iw + window_slot(s_idx, j);
-          const int k = pk[r];  // j < te <= tmax
-          const int w = pw[r];
+      for (int q = 0; q < kPer; q++) {
+        const long long j = j0 + q;
+        ok[q] = false;
+        it[q] = make_int4(0, 0, 0, 0);
+        if (j >= t0 && j < te) {
+          while (j >= nxt) nxt = W[++slot + 1];  // stops below kWin (j < W[kWin])
+          const int i = iw + slot;
+          const int k = pk[q];
+          const int w = pw[q];
           neg |= w < 0;
           if (k > i && (HEAVY ? w > thr : w <= thr)) {
             if (HEAVY) {
-              // after k_flatten every vertex points at its root, and no
-              // union runs during this kernel
               if (!(giant && member(gbits, i) && member(gbits, k))) {
                 const int a = ld(&parent[i]), b = ld(&parent[k]);
-                ok[r] = a != b;
-                it[r] = make_int4(a, b, w, (int)j);
+                ok[q] = a != b;
+                it[q] = make_int4(a, b, w, (int)j);
               }
             } else {
-              ok[r] = true;
-              it[r] = make_int4(i, k, w, (int)j);
+              ok[q] = true;
+              it[q] = make_int4(i, k, w, (int)j);
             }
           }
         }
       }
       if (neg) __hip_atomic_fetch_or(flags, kFlagNegative, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      append(A, it, ok, wl, cnt, cap, flags);
+      append_reg(A, fill, par, it, ok, wl, cnt, cap, flags);
     }
-    const long long niw = (long long)iw + (te < s_idx[kWin] ? window_slot(s_idx, te) : kWin);
-    __syncthreads();
+    const long long niw = (long long)iw + (te < W[kWin] ? window_slot(W, te) : kWin);
     t0 = te;
     iw = (int)std::min<long long>(niw, n);
+    par ^= 1;
   }
-  flush(A, wl, cnt, cap, flags);
+  flush_reg(A, fill, wl, cnt, cap, flags);
 }
 
 // Proposes `key` to the minimum of `root` for the active lanes.  The lanes
@@ -369,6 +450,45 @@ __device__ __forceinline__ void propose(unsigned long long* minv, int root, unsi
   if (lane == first) min64(&minv[r0], k);
   const unsigned long long rest = act & ~__ballot(same);
   if ((rest >> lane) & 1ull) min64(&minv[root], key);
+}
+
+// propose(), and then the remaining lanes also combine runs of neighbouring
+// lanes with one root: k_first lists a vertex's entries next to each other,
+// so in the first round of a phase the lanes of a wavefront hold runs of
+// one source vertex, and without this every lane of a run sends its own
+// test and atomic to the same address (the first rounds: R-MAT-24 1.03 ->
+// 0.90 ms, grid 256^3 1.26 -> 0.96 ms).  The later rounds' lists hold few
+// such runs, and k_link became slower with it (3.49 -> 4.40 ms).
+__device__ __forceinline__ void propose_runs(unsigned long long* minv, int root, unsigned long long key,
+                                        bool active) {
+  const unsigned long long act = __ballot(active);
+  if (!act) return;
+  const int lane = lane_id();
+  const int first = __ffsll((long long)act) - 1;
+  const int r0 = __shfl(root, first);
+  const bool same = active && root == r0;
+  unsigned long long k = same ? key : kNoEdge;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(k, d);
+    k = o < k ? o : k;
+  }
+  if (lane == first) min64(&minv[r0], k);
+  const unsigned long long rest = act & ~__ballot(same);
+  if (!rest) return;
+  // runs of neighbouring lanes with one root (list entries of one vertex):
+  // a suffix minimum within each run, proposed by the run's first lane
+  const bool mine = (rest >> lane) & 1ull;
+  k = mine ? key : kNoEdge;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long o = __shfl_down(k, d);
+    const int ro = __shfl_down(root, d);
+    if (lane + d < 64 && ro == root && o < k) k = o;
+  }
+  const int rp = __shfl_up(root, 1);
+  const bool prev = lane > 0 && ((rest >> (lane - 1)) & 1ull) && rp == root;
+  if (mine && !prev) min64(&minv[root], k);
 }
 
 // kernel1 (:240-256): keeps the listed edges whose ends are still in
@@ -432,8 +552,8 @@ __global__ __launch_bounds__(kThreads) void k_propose(const int4* in, const unsi
     const bool ok = i < total;
     const int4 e = ok ? in[i] : make_int4(0, 0, 0, 0);
     const unsigned long long key = key_of(e);
-    propose(minv, e.x, key, ok);
-    propose(minv, e.y, key, ok);
+    propose_runs(minv, e.x, key, ok);
+    propose_runs(minv, e.y, key, ok);
   }
 }
 
@@ -682,8 +802,9 @@ struct Runner {
     const long long tiles = (m + kTile - 1) / kTile;
     const int nb = (int)std::max<long long>(1, std::min<long long>(blocks, tiles));
     const long long per = (tiles + nb - 1) / nb * kTile;
+    const bool vec = ((reinterpret_cast<uintptr_t>(nlist) | reinterpret_cast<uintptr_t>(eweight)) & 15) == 0;
     k_first<HEAVY><<<nb, kThreads, 0, s>>>(n, m, nindex, nlist, eweight, L.parent, L.gbits, L.ctrl, L.wl[0],
-                                           L.cap, per);
+                                           L.cap, per, vec);
     Ctrl c;
     int rc = read_ctrl(&c);
     if (rc) return rc;

@@ -192,6 +192,37 @@ def test_gpu_mst_device_thresholds_agree(cuda):
 
 
 @pytest.mark.gpu
+def test_gpu_mst_on_arrays_not_16_byte_aligned(cuda):
+    # the list build loads entries and weights as 16-byte vectors only when
+    # both arrays are 16-byte aligned; caller arrays at other 4-byte offsets
+    # take the per-entry loads and give the same forest
+    import ctypes as C
+
+    import torch
+
+    g = workloads.rmat(13, 8, 6)
+    w = mst.mst_weights(g)
+    want = oracle.mst(g.nindex, g.nlist, w)
+    dev = torch.device("cuda:0")
+    L = N.mst_lib()
+    wb = int(L.eclmst_workspace_bytes(g.nodes, g.edges))
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    ni = torch.from_numpy(np.ascontiguousarray(g.nindex)).to(dev)
+    for off_list, off_w in ((1, 0), (0, 3), (2, 1)):
+        announce(f"mst with nlist at +{4 * off_list} B, weights at +{4 * off_w} B")
+        nl = torch.zeros(g.edges + 4, dtype=torch.int32, device=dev)
+        ww = torch.zeros(g.edges + 4, dtype=torch.int32, device=dev)
+        nl[off_list:off_list + g.edges] = torch.from_numpy(g.nlist).to(dev)
+        ww[off_w:off_w + g.edges] = torch.from_numpy(w).to(dev)
+        out = torch.zeros(g.edges, dtype=torch.uint8, device=dev)
+        ms = C.c_float(0.0)
+        N.check_mst(L.eclmst_run_device(g.nodes, g.edges, ni.data_ptr(), nl[off_list:].data_ptr(),
+                                        ww[off_w:].data_ptr(), out.data_ptr(), work.data_ptr(), wb, -1,
+                                        torch.cuda.current_stream().cuda_stream, C.byref(ms)))
+        assert np.array_equal(out.cpu().numpy(), want), (off_list, off_w)
+
+
+@pytest.mark.gpu
 def test_gpu_mst_medium_rmat(cuda):
     g = workloads.rmat(18, 16, 5)
     w = mst.mst_weights(g)
