@@ -370,6 +370,14 @@ __device__ __forceinline__ int hook_edge(int vstat, int u, int s, int* nstat, in
 // still unfinished after kLaneScan entries are finished cooperatively by
 // the whole wavefront, 64 entries per step, the first smaller entry being
 // the lowest set lane of a ballot.
+// With filter 2 the scan past the first two entries is left out (R-MAT-24
+// init 0.134 -> 0.118 ms, grid 256^3 0.116 -> 0.099 ms): a vertex whose
+// smaller neighbours lie further on keeps its own ID as label, which is a
+// valid forest (every label points to an ancestor), and its entries are
+// hooked later like any other vertex's.  Its label is then not the
+// reference's init value, and the labels that come out are still the
+// component minima.  In sorted lists (the usual case) nothing is lost: a
+// list whose first entry is larger than its vertex holds no smaller one.
 // The same pass bins the vertices by degree (the reference does this in
 // compute1, :116-127) into 64-bit ballot words per wavefront: vertices with
 // a smaller neighbour, vertices with a sampled hook, and low / medium / high
@@ -403,7 +411,10 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   // the positions one step past it must not wrap
   const unsigned uend = (unsigned)end;
   unsigned pos = (unsigned)beg + 2u;
-  bool scan = live && m == v && pos < uend;
+  // filters 0 and 1 bin the vertices by the smaller neighbour found here,
+  // so they need the whole scan; filter 2 bins every vertex with a list
+  const bool full = p.filter != 2;
+  bool scan = full && live && m == v && pos < uend;
   while (__ballot(scan)) {
     if (scan) {
       int x[8];
@@ -415,7 +426,7 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
       scan = m == v && pos < uend && pos < (unsigned)beg + kLaneScan;
     }
   }
-  const bool longscan = live && m == v && pos < uend;
+  const bool longscan = full && live && m == v && pos < uend;
   unsigned long long pending = __ballot(longscan);
   while (pending) {
     const int src = __ffsll((long long)pending) - 1;
@@ -458,7 +469,12 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const bool low = binned && deg <= p.thread_max;
   const bool mid = deg > p.thread_max && deg <= p.wave_max;
   const bool hub = deg > p.wave_max;
-  const unsigned long long actmask = __ballot(active);
+  // under filter 2 actbits serves one purpose: the work lists of a graph
+  // without a sampled largest tree, whose vertices hook the entries below
+  // them; a vertex not scanned past its first two entries may have such
+  // entries and is listed too (it hooks nothing if it has none)
+  const bool maybe = !full && live && m == v && pos < uend;
+  const unsigned long long actmask = __ballot(active || maybe);
   const unsigned long long smpmask = __ballot(sampled);
   const unsigned long long lowmask = __ballot(low);
   const unsigned long long midmask = __ballot(mid);

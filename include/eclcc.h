@@ -160,7 +160,9 @@ int eclcc_run_device_profiled(int32_t nodes, int32_t edges, const int32_t* d_nin
 
 /* Test hook: runs only the init kernel (ECL-CC-atomic.cu:69-88) so its
    closed form can be checked: d_nstat[v] = first neighbour < v in list
-   order, else v. */
+   order, else v.  (The full pass with giant_filter 2 looks at the first two
+   entries only and leaves v where neither is smaller; it runs init as
+   here with giant_filter 0 and 1.) */
 int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                         const int32_t* d_nlist, int32_t* d_nstat, void* d_work, size_t work_bytes,
                         void* stream);
