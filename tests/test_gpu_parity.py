@@ -138,6 +138,33 @@ def test_init_finds_the_smaller_neighbour_at_every_list_position(cuda):
     check(g, dg.labels())
 
 
+def test_filter2_without_a_largest_tree_on_unsorted_lists(cuda):
+    # 2^16 vertices in components of four: the 256 evenly spaced samples of
+    # pick all fall into different components, so no largest tree is found
+    # and the work lists are the possibly active vertices.  Vertex 4c+1 has
+    # the list [4c+2, 4c+3, 4c]: its only smaller neighbour sits past the two
+    # entries init looks at under filter 2, so it starts as its own root and
+    # must hook 4c from compute
+    from eclcc.graph import Graph
+
+    n = 1 << 16
+    c = np.arange(n // 4) * 4
+    src = np.concatenate([c + 1, c + 1, c + 1])
+    dst = np.concatenate([c + 2, c + 3, c])
+    g = Graph.from_edges(n, src, dst)
+    nl = g.nlist.copy()
+    for v in c + 1:
+        b, e = g.nindex[v], g.nindex[v + 1]
+        assert e - b == 3
+        nl[b:e] = [v + 1, v + 2, v - 1]
+    g.nlist = nl
+    announce("filter 2, no largest tree, smaller neighbour at list position 2")
+    for kw in (dict(), dict(sample_shift=2), dict(giant_filter=1), dict(giant_filter=0)):
+        labels, _ = compute_timed(g, default_params(**kw))
+        check(g, labels)
+        assert np.array_equal(labels, np.repeat(c, 4)), kw
+
+
 def test_device_runs_are_repeatable(cuda):
     g = workloads.rmat(16, 16, 12)
     announce(f"repeat {g.name}")
