@@ -151,7 +151,8 @@ def plain(args) -> dict:
     hubs, tasks = dg.hub_stats()
     # roofline of the kernel that takes the most time, on its own algorithmic bytes
     kb = roofline.kernel_bytes(g.nindex, g.nlist, dg.params.thread_max_deg, dg.params.wave_max_deg,
-                               dg.params.giant_filter, dg.params.sample_shift)
+                               dg.params.giant_filter, dg.params.sample_shift,
+                               dg.params.sample_min_deg)
     dom = max((k for k in KERNELS if kb[k]), key=lambda k: kernel_ms[k])
     dom_gbs = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9
     out = {

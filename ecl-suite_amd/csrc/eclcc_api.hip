@@ -83,6 +83,8 @@ int resolve_tuning(const eclcc_params* in, Tuning* out) {
   t.path_mode = p.path_mode;
   if (p.sample_shift < 0 || p.sample_shift > 4) return set_error(ECLCC_ERR_ARG, "sample_shift must be in [0, 4]");
   t.sample_shift = p.sample_shift;
+  if (p.sample_min_deg < 0) return set_error(ECLCC_ERR_ARG, "sample_min_deg must be >= 0");
+  t.sample_min_deg = p.sample_min_deg;
   if (t.thread_max < 0 || t.wave_max < t.thread_max)
     return set_error(ECLCC_ERR_ARG, "need 0 <= thread_max_deg <= wave_max_deg");
   if (t.wave_max < 1) return set_error(ECLCC_ERR_ARG, "wave_max_deg must be >= 1");
@@ -165,6 +167,7 @@ void fill_params(KParams* p, const Workspace& w, const Tuning& t) {
   p->filter = t.filter;
   p->path_mode = t.path_mode;
   p->sample_shift = t.sample_shift;
+  p->sample_min_deg = t.sample_min_deg;
 }
 
 }  // namespace eclcc
@@ -305,8 +308,14 @@ int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, 
     // skewed: the meeting walks climb long shared chains below the hubs'
     // trees, which halving shortens (R-MAT-24: 0.500 -> 0.491 ms); where
     // the walks share a few top labels the halving stores to them contend
-    // (ER-65536: 0.060 -> 0.091 ms)
+    // (ER-65536: 0.060 -> 0.091 ms).  The second-entry hooks go to lists of
+    // 16 entries or more: a shorter list left outside the largest tree is
+    // one thread's work in compute1, and most sampled vertices of a skewed
+    // graph have short lists (R-MAT-24 0.472 -> 0.448 ms, R-MAT-22 0.237
+    // -> 0.214, R-MAT-22 edge factor 8 0.184 -> 0.178, R-MAT-21 edge
+    // factor 4 0.175 -> 0.150; grids lose, 256^3 0.929 -> 1.08 ms at 4)
     p->path_mode = 2;
+    p->sample_min_deg = 16;
   }
   return ECLCC_OK;
 }
@@ -697,6 +706,7 @@ int eclcc_batch_run_single(eclcc_batch* b, int32_t g, void* stream) try {
   prm.giant_filter = b->tuning.filter;
   prm.path_mode = b->tuning.path_mode;
   prm.sample_shift = b->tuning.sample_shift;
+  prm.sample_min_deg = b->tuning.sample_min_deg;
   return eclcc_run_device(d.n, d.m, d.nidx, d.nlist, d.nstat, b->d_work1, b->work1_bytes, &prm,
                           stream);
 } catch (...) {

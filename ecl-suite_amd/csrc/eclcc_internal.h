@@ -82,6 +82,7 @@ struct KParams {
                   // component; 2: also leave the lists of its members unread
   int path_mode;  // 0: pointer jumping at every step (reference); 1: root jump
   int sample_shift;  // second-entry hooks taken by 1 in 2^sample_shift vertices
+  int sample_min_deg;  // ... whose lists hold at least this many entries
 };
 
 // Workspace carved out of one device buffer.
@@ -109,7 +110,8 @@ struct Shape {
 
 // Resolved, validated parameters.
 struct Tuning {
-  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter, path_mode, sample_shift;
+  int thread_max, wave_max, hub_chunk, hub_blocks, block_threads, filter, path_mode, sample_shift,
+      sample_min_deg;
 };
 
 size_t workspace_bytes(long long nslots, long long hub_cap, long long ngraphs);

@@ -454,8 +454,10 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const int hooked = active ? x1 : x0;
   // with sample_shift > 0 only 1 in 2^sample_shift of the vertices with a
   // smaller neighbour (by a multiplicative hash of the ID) take theirs
-  const bool pick = p.sample_shift == 0 ||
-                    ((unsigned)v * 0x9E3779B1u) >> (32 - p.sample_shift) == 0u;
+  // and only those with at least sample_min_deg entries
+  const bool pick = (p.sample_shift == 0 ||
+                     ((unsigned)v * 0x9E3779B1u) >> (32 - p.sample_shift) == 0u) &&
+                    end - beg >= p.sample_min_deg;
   const bool sampled = active ? (x1 < v && pick) : (live && x0 != INT_MAX && x0 != v);
   if (live) {
     G.nstat[v] = m;  // no other thread reads labels during init

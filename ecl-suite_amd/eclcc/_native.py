@@ -38,6 +38,7 @@ class EclccParams(C.Structure):
         ("giant_filter", i32),
         ("path_mode", i32),
         ("sample_shift", i32),
+        ("sample_min_deg", i32),
     ]
 
     def as_dict(self) -> dict:

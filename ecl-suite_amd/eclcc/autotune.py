@@ -113,6 +113,7 @@ def load(key: str, path: Path = TABLE) -> Optional[N.EclccParams]:
 PLAIN_SPACE: List[Dict[str, int]] = (
     [dict()]
     + [dict(sample_shift=s) for s in (0, 1, 2, 3)]
+    + [dict(sample_min_deg=d) for d in (0, 8, 16, 32)]
     + [dict(path_mode=m) for m in (0, 1, 2)]
     + [dict(thread_max_deg=t) for t in (4, 8, 32, 64)]
     + [dict(wave_max_deg=w) for w in (256, 512, 1024, 4096, 8192)]

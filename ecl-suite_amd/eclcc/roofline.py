@@ -19,7 +19,8 @@ A vertex is active when it has a smaller neighbour (only those hook,
 ECL-CC-atomic.cu:119-132).  The sample kernel hooks one entry per vertex:
 the second entry of an active vertex if that is smaller (with sample_shift
 s > 0 only for the active vertices whose multiplicative hash
-(v * 0x9E3779B1 mod 2^32) has its top s bits clear), the first entry of an
+(v * 0x9E3779B1 mod 2^32) has its top s bits clear, and with
+sample_min_deg d > 0 only for those with at least d entries), the first entry of an
 inactive vertex with a neighbour.  Labels are counted as read by the
 kernels that need them (sample, mark, flatten), not only as written.  The bins split the active
 vertices by degree:
@@ -50,7 +51,7 @@ def pass_bytes(n: int, m: int) -> int:
 
 
 def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
-         sample_shift: int = 0) -> Dict[str, tuple]:
+         sample_shift: int = 0, sample_min_deg: int = 0) -> Dict[str, tuple]:
     """(vertices, entries) of the active vertices in each degree bin."""
     n = nindex.shape[0] - 1
     deg = np.diff(nindex).astype(np.int64)
@@ -70,6 +71,7 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
     if sample_shift:
         h = (ids.astype(np.uint64) * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)
         pick = (h >> np.uint64(32 - sample_shift)) == 0
+    pick &= deg >= sample_min_deg
     sampled = np.where(active, (second < ids) & pick, has & (first != ids))
     low = active & (deg <= thread_max)
     mid = active & (deg > thread_max) & (deg <= wave_max)
@@ -84,9 +86,10 @@ def bins(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
 
 
 def kernel_bytes(nindex: np.ndarray, nlist: np.ndarray, thread_max: int, wave_max: int,
-                 giant_filter: int = 1, sample_shift: int = 0) -> Dict[str, Optional[int]]:
+                 giant_filter: int = 1, sample_shift: int = 0,
+                 sample_min_deg: int = 0) -> Dict[str, Optional[int]]:
     n = nindex.shape[0] - 1
-    b = bins(nindex, nlist, thread_max, wave_max, sample_shift)
+    b = bins(nindex, nlist, thread_max, wave_max, sample_shift, sample_min_deg)
     per_bin: Dict[str, Optional[int]] = {k: 4 * e + 8 * v for k, (v, e) in b.items()}
     if giant_filter == 2:
         per_bin.update(low=None, mid=None, hub=None)

@@ -81,6 +81,13 @@ typedef struct eclcc_params {
                              them, picked by a hash of the ID; 0 (default): by all.
                              Vertices without a smaller neighbour always take theirs.
                              The labels do not depend on it (0..4) */
+  int32_t sample_min_deg; /* and only those whose list has at least this many entries
+                             (0, the default: every length).  A short list left
+                             outside the largest tree costs compute one thread; a
+                             long one costs a wavefront, so on skewed graphs the
+                             sampled hooks are best spent on long lists (R-MAT-24
+                             with 16: 5 % less time per pass).  Labels do not
+                             depend on it (>= 0) */
 } eclcc_params;
 
 /* Fills the built-in defaults (thresholds re-derived for 64-lane waves). */
@@ -96,9 +103,10 @@ void eclcc_params_default(eclcc_params* p);
    ms per pass on MI355X), and from 2^23 vertices on also halve the paths
    of their meeting walks (path_mode 2; Erdos-Renyi n = 2^24, degree 6:
    1.11 -> 0.90 ms); skewed degree distributions (R-MAT, power-law)
-   keep them all and halve the paths their meeting walks climb
-   (sample_shift 0, path_mode 2; R-MAT-24: 0.500 -> 0.491 ms, 0.59 with
-   sample_shift 1).  nindex is host memory; at most 8192 of its entries are
+   keep them all for the vertices with at least 16 entries and halve the
+   paths their meeting walks climb (sample_shift 0, sample_min_deg 16,
+   path_mode 2; R-MAT-24: 0.500 -> 0.491 ms with the halving, 0.475 ->
+   0.448 ms with the length limit, 0.59 with sample_shift 1).  nindex is host memory; at most 8192 of its entries are
    read.  The labels do not depend on the choice. */
 int eclcc_params_for_graph(int32_t nodes, int64_t edges, const int32_t* nindex, eclcc_params* p);
 

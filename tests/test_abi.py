@@ -39,12 +39,13 @@ def test_params_defaults_and_workspace():
     assert p.giant_filter == 2 and p.path_mode == 1
     for f in (0, 1, 2):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(giant_filter=f))) > 0
-    assert p.sample_shift == 0
+    assert p.sample_shift == 0 and p.sample_min_deg == 0
+    assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(sample_min_deg=1 << 30))) > 0
     for sh in range(5):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(eclcc.default_params(sample_shift=sh))) > 0
     for bad in (eclcc.default_params(giant_filter=3), eclcc.default_params(giant_filter=-1),
                 eclcc.default_params(path_mode=3), eclcc.default_params(sample_shift=5),
-                eclcc.default_params(sample_shift=-1)):
+                eclcc.default_params(sample_shift=-1), eclcc.default_params(sample_min_deg=-1)):
         assert L.eclcc_workspace_bytes(1000, 10000, C.byref(bad)) == 0
 
 
@@ -103,22 +104,23 @@ def test_label_accesses_go_through_atomic_helpers():
 def test_params_for_graph_picks_by_degree_skew():
     # graphs of 2^20 vertices or more: even degrees (grid, Erdos-Renyi) take
     # half of the sampled hooks, and from 2^23 vertices the halving walks
-    # too; a heavy tail (R-MAT) takes the halving walks; smaller graphs and
+    # too; a heavy tail (R-MAT) takes the halving walks and spends its
+    # second-entry hooks on lists of 16 entries or more; smaller graphs and
     # the rest of the parameters keep the defaults
     from eclcc import Graph, params_for_graph, workloads
 
     d = eclcc.default_params().as_dict()
-    for g, shift, mode in ((workloads.grid3d(104, 0.6, 1), 1, 1), (Graph.er(1 << 20, 4 << 20, 3), 1, 1),
-                           (workloads.rmat(20, 16, 2), 0, 2), (workloads.rmat(16, 16, 2), 0, 1),
-                           (workloads.er65536(1), 0, 1), (workloads.grid3d(32, 0.6, 1), 0, 1)):
+    for g, shift, mode, mind in ((workloads.grid3d(104, 0.6, 1), 1, 1, 0), (Graph.er(1 << 20, 4 << 20, 3), 1, 1, 0),
+                                 (workloads.rmat(20, 16, 2), 0, 2, 16), (workloads.rmat(16, 16, 2), 0, 1, 0),
+                                 (workloads.er65536(1), 0, 1, 0), (workloads.grid3d(32, 0.6, 1), 0, 1, 0)):
         p = params_for_graph(g).as_dict()
-        assert p == dict(d, sample_shift=shift, path_mode=mode), g.name
+        assert p == dict(d, sample_shift=shift, path_mode=mode, sample_min_deg=mind), g.name
     # a large star is skewed; a large empty graph is not; small ones keep the defaults
     import graphs as G
 
     # (the hub at a sampled position: the choice reads 4096 evenly spaced degrees)
     star = Graph.from_edges(1 << 20, np.zeros((1 << 20) - 1, dtype=np.int32), np.arange(1, 1 << 20))
-    assert params_for_graph(star).as_dict() == dict(d, path_mode=2)
+    assert params_for_graph(star).as_dict() == dict(d, path_mode=2, sample_min_deg=16)
     assert params_for_graph(G.empty(1 << 20)).as_dict() == dict(d, sample_shift=1)
     assert params_for_graph(G.empty((1 << 23) - 1)).as_dict() == dict(d, sample_shift=1)
     assert params_for_graph(G.empty(1 << 23)).as_dict() == dict(d, sample_shift=1, path_mode=2)

@@ -188,9 +188,10 @@ def test_server_bulk_device_requests(cuda):
 @pytest.mark.parametrize("kw", [dict(), dict(giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=64),
                                 dict(giant_filter=1, sample_shift=3),
                                 dict(giant_filter=2, thread_max_deg=0, wave_max_deg=1, path_mode=0),
-                                dict(path_mode=2, sample_shift=2), dict(path_mode=2, giant_filter=0)],
+                                dict(path_mode=2, sample_shift=2), dict(path_mode=2, giant_filter=0),
+                                dict(path_mode=2, sample_min_deg=16)],
                          ids=["default", "f0-small-bins", "f1-shift3", "f2-all-hubs", "halve-shift2",
-                              "halve-f0"])
+                              "halve-f0", "halve-long-lists"])
 def test_batch_of_irregular_graphs(cuda, kw):
     # unsorted lists, duplicates, self-loops, stars with the hub anywhere,
     # permuted paths, shuffled cliques, sizes from 1 to 3000 (most not a

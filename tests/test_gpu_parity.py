@@ -79,6 +79,9 @@ PARAMS = [
     dict(path_mode=2),
     dict(path_mode=2, giant_filter=0, thread_max_deg=2, wave_max_deg=40, hub_chunk=128),
     dict(path_mode=2, giant_filter=1, sample_shift=2),
+    dict(sample_min_deg=16, path_mode=2),
+    dict(sample_min_deg=3, sample_shift=1, giant_filter=1),
+    dict(sample_min_deg=1 << 30),
 ]
 
 

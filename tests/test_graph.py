@@ -270,19 +270,20 @@ def test_sampled_vertex_count_matches_the_rule():
     from eclcc import roofline
 
     g = workloads.rmat(11, 8, seed=5)
-    for shift in (0, 2):
+    for shift, min_deg in ((0, 0), (2, 0), (0, 16), (1, 5)):
         want = 0
         for v in range(g.nodes):
             lst = g.nlist[g.nindex[v]:g.nindex[v + 1]]
             if len(lst) == 0:
                 continue
-            if lst.min() < v:  # active: its second entry, if smaller (1 in 2^shift of them)
+            if lst.min() < v:  # active: its second entry, if smaller (1 in 2^shift of
+                # them, and only with at least min_deg entries)
                 picked = shift == 0 or ((v * 0x9E3779B1) & 0xFFFFFFFF) >> (32 - shift) == 0
-                want += int(len(lst) >= 2 and lst[1] < v and picked)
+                want += int(len(lst) >= 2 and lst[1] < v and picked and len(lst) >= min_deg)
             else:              # inactive: its first entry
                 want += int(lst[0] != v)
-        assert roofline.bins(g.nindex, g.nlist, 16, 2048, shift)["sampled"][0] == want
-        assert roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048, 2, shift)["sample"] == 16 * want
+        assert roofline.bins(g.nindex, g.nlist, 16, 2048, shift, min_deg)["sampled"][0] == want
+        assert roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048, 2, shift, min_deg)["sample"] == 16 * want
     assert roofline.kernel_bytes(g.nindex, g.nlist, 16, 2048)["mark"] == 4 * g.nodes + g.nodes // 8
 
 
