@@ -842,9 +842,31 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
 // outside R's tree (each then hooks all of its entries), or, in a graph
 // without a sampled root, the vertices that have a smaller neighbour (each
 // hooks the entries below it, as the reference does); the high-degree ones
-// among them are few and go to compute2, which takes lists of any length.
-// Otherwise mark counts each word's high-degree vertices and their edge
-// tasks for the scan.
+// among them are few, and mark lists them with their edge tasks at once
+// (list_hub) for compute2's wavefronts.  Otherwise mark counts each word's
+// high-degree vertices and their edge tasks for the scan.
+
+// Filter 2: appends a high-degree vertex to the hub list.  One 64-bit atomic
+// add of (1 | tasks << 32) hands out its index and its first task number
+// together, so the list is ordered by first task in whatever order the adds
+// arrive, and no scan is needed.  Such vertices lie outside the largest tree
+// and are rare (none on R-MAT-24, whose hubs all join it), so the counter is
+// not contended.
+__device__ __forceinline__ void list_hub(const KParams& p, const GraphRef& G, long long slot,
+                                         int v) {
+  const int beg = G.nidx[v];
+  const int end = G.nidx[v + 1];
+  const unsigned tasks = task_count(end - beg, p.hub_chunk);
+  const unsigned long long old =
+      __hip_atomic_fetch_add(p.hubctr, 1ull | ((unsigned long long)tasks << 32), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  const long long idx = (long long)(unsigned)old;
+  if (idx < p.hub_cap) {
+    p.hubs[idx] = HubEntry{slot, (long long)(old >> 32), beg, end};
+  } else {
+    __hip_atomic_fetch_or(p.flags, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // One graph: one lane per vertex (four walks per lane leave too few
 // wavefronts in flight on one graph: R-MAT-24 0.108 vs 0.113 ms).
@@ -868,8 +890,9 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   const unsigned long long gw = __ballot(member);
   unsigned long long keep = ~0ull;
   if (p.filter == 2) keep = R >= 0 ? ~gw : act;
-  // filter 2: the few high-degree vertices left join compute2's lists, and
-  // the hub scan and compute3 are not launched
+  // filter 2: the few high-degree vertices left are listed for compute2's
+  // hub tasks; the hub scan and compute3 are not launched
+  if (p.filter == 2 && ((hub & keep) >> lane) & 1ull) list_hub(p, G, slot, v);
   const unsigned long long hb = p.filter == 2 ? 0ull : hub & keep;
   unsigned long long info = 0;
   if (hb) {
@@ -887,7 +910,7 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
     p.hubinfo[w] = info;
     if (p.filter == 2) {
       p.lowbits[w] = low & keep;
-      p.midbits[w] = (mid | hub) & keep;
+      p.midbits[w] = mid & keep;
       p.hubbits[w] = 0ull;
     }
   }
@@ -955,8 +978,10 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
     const unsigned long long gw = __ballot(member);
     unsigned long long keep = ~0ull;
     if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
-    // filter 2: the few high-degree vertices left join compute2's lists, and
-    // the hub scan and compute3 are not launched
+    // filter 2: the few high-degree vertices left are listed for compute2's
+    // hub tasks; the hub scan and compute3 are not launched
+    if (p.filter == 2 && ((p.hubbits[w] & keep) >> lane) & 1ull)
+      list_hub(p, G, ws0 + k * 64 + lane, v[k]);
     const unsigned long long hb = p.filter == 2 ? 0ull : p.hubbits[w] & keep;
     unsigned long long info = 0;
     if (hb) {
@@ -974,7 +999,7 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
       p.hubinfo[w] = info;
       if (p.filter == 2) {
         p.lowbits[w] &= keep;
-        p.midbits[w] = (p.midbits[w] | p.hubbits[w]) & keep;
+        p.midbits[w] &= keep;
         p.hubbits[w] = 0ull;
       }
     }
@@ -1021,21 +1046,14 @@ __device__ __forceinline__ void flush_wave(const int2* surv, int cnt, int* ns, i
   }
 }
 
+// The listed vertices of one 256-slot chunk (one graph), by one wavefront;
+// `owner`, `list`, `lbase`, `lroot`, `lvtx` and `surv` are its LDS tables.
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
-  __shared__ unsigned char s_owner[4][kWaveWindow];
-  __shared__ unsigned char s_list[4][kSlotAlign];
-  __shared__ int s_base[4][64];
-  __shared__ int s_root[4][64];
-  __shared__ int s_vtx[4][64];
-  __shared__ int2 s_surv[4][kWaveSurvivors];
-  // one 256-slot chunk (one graph) per wavefront
-  const long long ws0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
-  if (ws0 >= p.nslots) return;
+__device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
+                                                unsigned char* owner, unsigned char* list,
+                                                int* lbase, int* lroot, int* lvtx, int2* surv) {
   const long long nwords = (p.nslots + 63) / 64;
   const int lane = lane_id();
-  const int wv = uniform((int)(threadIdx.x >> 6));
-  unsigned char* const list = s_list[wv];
   int cnt = 0;  // wave-uniform
 #pragma unroll
   for (int i = 0; i < kSlotAlign / 64; i++) {
@@ -1053,8 +1071,6 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
   const GraphRef G = resolve<BATCH>(p, ws0);
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
-  unsigned char* const owner = s_owner[wv];
-  int2* const surv = s_surv[wv];
   const int R = p.filter ? p.giant[G.g] : -1;
   const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
   int nsurv = 0;  // wave-uniform
@@ -1080,9 +1096,9 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
     const int incl = (int)wave_incl_sum((unsigned)deg);
     const int off = incl - deg;
     const int total = __shfl(incl, 63);
-    s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
-    s_root[wv][lane] = root;
-    s_vtx[wv][lane] = v;
+    lbase[lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
+    lroot[lane] = root;
+    lvtx[lane] = v;
     // window positions are unsigned: the lists of one wavefront may hold
     // nearly 2^31 entries, and a step past the last must not wrap
     const unsigned utotal = (unsigned)total;
@@ -1106,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
           const unsigned e = e0 + k * 64 + lane;
           const bool valid = e < wend;
           src[k] = valid ? owner[e - w0] : 0;
-          u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
+          u[k] = valid ? ld_stream(&nl[lbase[src[k]] + e]) : INT_MAX;
         }
         unsigned keep = 0;
         if (skip) {
@@ -1115,7 +1131,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
           // owner is noted and hooked to R once, after its list
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
+            keep |= (u[k] != INT_MAX && u[k] != lvtx[src[k]]) ? (1u << k) : 0u;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) {
@@ -1125,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
           }
         } else {
 #pragma unroll
-          for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
+          for (int k = 0; k < 4; k++) keep |= (lvtx[src[k]] > u[k]) ? (1u << k) : 0u;
           if (mk) {
             unsigned in = 0;
 #pragma unroll
@@ -1153,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
           int at = nsurv + (int)(pos - c);
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
+            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], lroot[src[k]]);
           nsurv += wtot;
         }
       }
@@ -1170,6 +1186,134 @@ __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
     }
   }
   if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
+}
+
+// Filter 2: the lists of the high-degree vertices that mark listed (outside
+// the largest tree, or, in a graph without one, those that may have a
+// smaller neighbour), in tasks of hub_chunk entries.  Task t goes to
+// wavefront t mod (the wavefronts of the launch), after that wavefront's own
+// chunk, so a long list is spread over many wavefronts and no launch is
+// added for the usual case of no such vertex (one load of the counter per
+// wavefront).  A task streams its entries with coalesced loads, four per
+// lane per step, stages the ones that need a hook in LDS as (neighbour, root
+// estimate) pairs and hooks them together, as the chunk lists above do;
+// entries into the largest tree stand for one hook to its root.  (Dealt over
+// the lanes of one wavefront with the chunk lists instead, a vertex with a
+// million neighbours outside the largest tree took 10.3 ms.)
+template <bool BATCH>
+__device__ __forceinline__ void hook_hub_tasks(const KParams& p, int2* surv, long long gw,
+                                               long long nw) {
+  const unsigned long long ctr =
+      __hip_atomic_load(p.hubctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const long long pushed = (long long)(unsigned)ctr;
+  const long long nhubs = pushed < p.hub_cap ? pushed : p.hub_cap;
+  const long long ntasks = (long long)(ctr >> 32);
+  if (gw >= ntasks || nhubs <= 0) return;
+  const int lane = lane_id();
+  long long k = 0;  // hubs[k].first_task <= t throughout (hubs[0].first_task is 0)
+  for (long long t = gw; t < ntasks; t += nw) {
+    // the last hub whose first task is <= t: 64 probes per step; first
+    // tasks rise strictly with the index, every listed list holding a task
+    long long hi = nhubs;
+    while (hi - k > 1) {
+      const long long step = (hi - k + 63) / 64;
+      const long long probe = k + (long long)lane * step;
+      const bool ok = probe < hi && p.hubs[probe].first_task <= t;
+      const unsigned long long b = __ballot(ok);  // lanes 0..j, lane 0 always
+      const int j = 63 - __clzll((long long)b);
+      k += (long long)j * step;
+      hi = k + step < hi ? k + step : hi;
+    }
+    const HubEntry h = p.hubs[k];
+    const GraphRef G = resolve<BATCH>(p, h.slot);
+    int* const ns = G.nstat;
+    const int v = (int)(h.slot - G.slot0);
+    const int Rg = p.giant[G.g];
+    const long long cb = (long long)h.beg + (t - h.first_task) * (long long)p.hub_chunk;
+    const long long ce = cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end;
+    int vroot = 0;
+    if (lane == 0) vroot = find_root(v, ns, p.path_mode);
+    vroot = __shfl(vroot, 0);
+    int nsurv = 0;  // wave-uniform
+    bool intree = false;
+    for (long long e0 = cb; e0 < ce; e0 += 256) {
+      int u[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const long long e = e0 + q * 64 + lane;
+        u[q] = e < ce ? ld_stream(&G.nlist[e]) : INT_MAX;
+      }
+      unsigned keep = 0;
+      if (Rg >= 0) {
+        // every entry; one into the largest tree is noted, not staged
+#pragma unroll
+        for (int q = 0; q < 4; q++) keep |= (u[q] != INT_MAX && u[q] != v) ? (1u << q) : 0u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (((keep >> q) & 1u) && slot_bit(p.gbits, G.slot0 + u[q])) {
+            keep &= ~(1u << q);
+            intree = true;
+          }
+        }
+      } else {
+        // no largest tree: the entries below v, as the reference hooks them
+#pragma unroll
+        for (int q = 0; q < 4; q++) keep |= u[q] < v ? (1u << q) : 0u;
+      }
+      const unsigned c = __popc(keep);
+      const unsigned pos = wave_incl_sum(c);
+      const int wtot = (int)__shfl(pos, 63);
+      if (wtot) {
+        if (nsurv + wtot > kWaveSurvivors) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          flush_wave(surv, nsurv, ns, p.path_mode);
+          nsurv = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        int at = nsurv + (int)(pos - c);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if ((keep >> q) & 1u) surv[at++] = make_int2(u[q], vroot);
+        nsurv += wtot;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // all entries of the task into the largest tree join the same two trees
+    if (__ballot(intree) && lane == 0) (void)hook_edge(vroot, Rg, lab_ld(&ns[Rg]), ns, p.path_mode);
+  }
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
+  __shared__ unsigned char s_owner[4][kWaveWindow];
+  __shared__ unsigned char s_list[4][kSlotAlign];
+  __shared__ int s_base[4][64];
+  __shared__ int s_root[4][64];
+  __shared__ int s_vtx[4][64];
+  __shared__ int2 s_surv[4][kWaveSurvivors];
+  const int wv = uniform((int)(threadIdx.x >> 6));
+  const long long gw = (long long)blockIdx.x * 4 + wv;
+  // one 256-slot chunk (one graph) per wavefront
+  const long long ws0 = gw * kSlotAlign;
+  if (ws0 < p.nslots)
+    hook_chunk_wave<BATCH>(p, ws0, s_owner[wv], s_list[wv], s_base[wv], s_root[wv], s_vtx[wv],
+                           s_surv[wv]);
+  if (p.filter == 2) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    hook_hub_tasks<BATCH>(p, s_surv[wv], gw, (long long)gridDim.x * 4);
+  }
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------

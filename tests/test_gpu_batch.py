@@ -215,6 +215,34 @@ def test_batch_of_irregular_graphs(cuda, kw):
             assert np.array_equal(lab, e), ("single", g.name)
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(wave_max_deg=64, hub_chunk=64), dict(thread_max_deg=4, wave_max_deg=8, hub_chunk=64, path_mode=2)],
+                         ids=["default", "wave64-chunk64", "wave8-chunk64-halve"])
+def test_batch_with_stars_outside_each_largest_tree(cuda, kw):
+    # graphs whose star centres lie outside their largest tree, between
+    # ordinary mix graphs: with filter 2 mark lists the centres of all graphs
+    # in one hub list, and a task's graph comes from its hub's slot
+    import graphs as G
+    import torch
+
+    gs = []
+    for i in range(6):
+        gs.append(G.er_with_stars(20_000 + 3000 * i, 80_000, 1 + i % 3, 4000 + 1000 * i, seed=40 + i))
+        gs += small_mix(2, seed=300 + i, n_hi=30000)
+    expect = [oracle.cc_labels(g.nindex, g.nlist) for g in gs]
+    announce(f"batch of {len(gs)} graphs with stars outside their largest trees, params {kw}")
+    with Batch(gs, default_params(**kw)) as b:
+        for rep in range(3):
+            b.run(use_graph=rep > 0)
+            torch.cuda.synchronize()
+            for g, e, lab in zip(gs, expect, b.all_labels()):
+                assert np.array_equal(lab, e), (rep, g.name)
+        for i in range(len(gs)):
+            b.run_single(i)
+        torch.cuda.synchronize()
+        for g, e, lab in zip(gs, expect, b.all_labels()):
+            assert np.array_equal(lab, e), ("single", g.name)
+
+
 def test_server_on_irregular_graphs_under_a_tight_budget(cuda):
     # requests of every irregular shape (down to one vertex) submitted at
     # once to a server whose small HBM budget forces many small batches:

@@ -269,6 +269,30 @@ def test_high_degree_vertices_outside_the_largest_tree(cuda, filt):
             check(g, compute(g, default_params(giant_filter=filt, **kw)))
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(hub_chunk=64), dict(hub_chunk=128, wave_max_deg=16, path_mode=0),
+                                dict(path_mode=2, sample_shift=1), dict(giant_filter=1)],
+                         ids=["default", "chunk64", "wave16-mode0", "mode2-shift1", "filter1"])
+def test_large_stars_outside_the_largest_tree(cuda, kw):
+    # three stars of 50,000 leaves beside a random graph of 200,000 vertices:
+    # with filter 2 mark lists the three centres (outside the largest tree)
+    # with their edge tasks, and compute2's wavefronts hook their lists in
+    # tasks of hub_chunk entries
+    from eclcc import params_for_graph
+
+    g = G.er_with_stars(200_000, 800_000, 3, 50_000, seed=5)
+    p = params_for_graph(g, **kw)
+    announce(f"{g.name} n={g.nodes} m={g.edges} params {kw}")
+    dg = DeviceGraph(g, "cuda:0", p)
+    for rep in range(2):
+        dg.nstat.fill_(-3)
+        dg.run()
+        check(g, dg.labels())
+    hubs, tasks = dg.hub_stats()
+    if p.giant_filter == 2:
+        chunk = p.hub_chunk
+        assert hubs >= 3 and tasks >= 3 * ((50_000 + chunk - 1) // chunk), (hubs, tasks)
+
+
 def test_cli_runs_like_the_reference_minibench(cuda, tmp_path):
     # the process face: `eclcc <file.egr>` prints the reference's lines, and
     # its analyzer reads the time with `time: +(\d+\.\d+) s`
