@@ -22,8 +22,9 @@
 //   compute2  medium degree: wavefront granularity
 //   compute3  high degree: block granularity, edge tasks staged through LDS
 //   flatten   labels := root (reference flatten)
-// With filter 2 hubscan and compute3 are not launched (mark hands the few
-// high-degree vertices left to compute2).  A batch runs the same sequence
+// With filter 2 hubscan is not launched: mark lists the few high-degree
+// vertices left outside the largest tree with their edge tasks itself, and
+// compute3 takes those tasks.  A batch runs the same sequence
 // over the slots of all its graphs (BATCH = true): sample and mark then
 // work per block list and per 256-slot chunk, with their blocks taken in a
 // spread order over the batch (spread_block).
@@ -543,21 +544,24 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
 // dense thread walks its vertices one after the other, and the last blocks
 // finish late.  Blocks of fewer words are slower on batches (1024 graphs,
 // sample with 8 / 16 / 32 words per block: +90 / +55 / +20 %): the blocks
-// that run together then cover fewer graphs (see spread_block).
+// that run together then cover fewer graphs (see spread_block).  A small
+// batch (the server's, of a few dozen graphs) takes fewer words per block,
+// `span`, so that the launch still has enough blocks to fill the GPU: at 64
+// words a batch of one ER graph with 65,536 vertices has 16 blocks.
 constexpr int kSampleWords = 64;
 
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride) {
+__global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride, int span) {
   static_assert(kSampleWords <= 64, "one word per lane of the first wavefront");
   __shared__ unsigned short s_list[kSampleWords * 64];
   __shared__ unsigned long long s_bits[kSampleWords];
   __shared__ int s_off[kSampleWords];
   __shared__ int s_cnt;
   const long long words = (p.nslots + 63) / 64;
-  const long long w0 = (long long)spread_block(stride) * kSampleWords;
-  if (threadIdx.x < 64) (void)gather_words(p.smpbits, w0, kSampleWords, words, s_bits, s_off, &s_cnt);
+  const long long w0 = (long long)spread_block(stride) * span;
+  if (threadIdx.x < 64) (void)gather_words(p.smpbits, w0, span, words, s_bits, s_off, &s_cnt);
   __syncthreads();
-  list_words(s_bits, s_off, kSampleWords, s_list);
+  list_words(s_bits, s_off, span, s_list);
   __syncthreads();
   const int cnt = s_cnt;
   for (int j = threadIdx.x; j < cnt; j += 256) {
@@ -843,15 +847,18 @@ __global__ __launch_bounds__(kPickSamples) void k_pick(KParams p) {
 // without a sampled root, the vertices that have a smaller neighbour (each
 // hooks the entries below it, as the reference does); the high-degree ones
 // among them are few, and mark lists them with their edge tasks at once
-// (list_hub) for compute2's wavefronts.  Otherwise mark counts each word's
-// high-degree vertices and their edge tasks for the scan.
+// (list_hub) for compute3.  Otherwise mark counts each word's high-degree
+// vertices and their edge tasks for the scan.
 
 // Filter 2: appends a high-degree vertex to the hub list.  One 64-bit atomic
 // add of (1 | tasks << 32) hands out its index and its first task number
 // together, so the list is ordered by first task in whatever order the adds
 // arrive, and no scan is needed.  Such vertices lie outside the largest tree
-// and are rare (none on R-MAT-24, whose hubs all join it), so the counter is
-// not contended.
+// and are rare (none on R-MAT-24, whose hubs all join it; about 40 tasks in
+// 1024 graphs of the batch mix), so the counter is not contended.  Before,
+// they joined compute2's chunk lists, where one wavefront dealt a whole hub
+// list over its lanes: a vertex with a million neighbours outside the
+// largest tree took 10.3 ms there.
 __device__ __forceinline__ void list_hub(const KParams& p, const GraphRef& G, long long slot,
                                          int v) {
   const int beg = G.nidx[v];
@@ -890,8 +897,7 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   const unsigned long long gw = __ballot(member);
   unsigned long long keep = ~0ull;
   if (p.filter == 2) keep = R >= 0 ? ~gw : act;
-  // filter 2: the few high-degree vertices left are listed for compute2's
-  // hub tasks; the hub scan and compute3 are not launched
+  // filter 2: the few high-degree vertices left are listed for compute3
   if (p.filter == 2 && ((hub & keep) >> lane) & 1ull) list_hub(p, G, slot, v);
   const unsigned long long hb = p.filter == 2 ? 0ull : hub & keep;
   unsigned long long info = 0;
@@ -938,6 +944,18 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
     v[k] = (int)(slot - G.slot0);
     live[k] = slot < p.nslots && v[k] < G.n;
   }
+  // the chunk's list words, loaded before the walks so that their latency
+  // hides behind them (no other wavefront touches these words during mark)
+  unsigned long long actw[K], loww[K], midw[K], hubw[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const long long w = (ws0 >> 6) + k;
+    const bool in = w < nwords;
+    actw[k] = in ? p.actbits[w] : 0ull;
+    loww[k] = in ? p.lowbits[w] : 0ull;
+    midw[k] = in ? p.midbits[w] : 0ull;
+    hubw[k] = in ? p.hubbits[w] : 0ull;
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) first[k] = (R >= 0 && live[k]) ? lab_ld(&ns[v[k]]) : 0;
 #pragma unroll
@@ -977,12 +995,10 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
     const bool member = R >= 0 && live[k] && (v[k] == R || (first[k] != v[k] && cur[k] == R));
     const unsigned long long gw = __ballot(member);
     unsigned long long keep = ~0ull;
-    if (p.filter == 2) keep = R >= 0 ? ~gw : p.actbits[w];
-    // filter 2: the few high-degree vertices left are listed for compute2's
-    // hub tasks; the hub scan and compute3 are not launched
-    if (p.filter == 2 && ((p.hubbits[w] & keep) >> lane) & 1ull)
-      list_hub(p, G, ws0 + k * 64 + lane, v[k]);
-    const unsigned long long hb = p.filter == 2 ? 0ull : p.hubbits[w] & keep;
+    if (p.filter == 2) keep = R >= 0 ? ~gw : actw[k];
+    // filter 2: the few high-degree vertices left are listed for compute3
+    if (p.filter == 2 && ((hubw[k] & keep) >> lane) & 1ull) list_hub(p, G, ws0 + k * 64 + lane, v[k]);
+    const unsigned long long hb = p.filter == 2 ? 0ull : hubw[k] & keep;
     unsigned long long info = 0;
     if (hb) {
       unsigned tasks = 0;
@@ -998,8 +1014,8 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
       p.gbits[w] = gw;
       p.hubinfo[w] = info;
       if (p.filter == 2) {
-        p.lowbits[w] &= keep;
-        p.midbits[w] &= keep;
+        p.lowbits[w] = loww[k] & keep;
+        p.midbits[w] = midw[k] & keep;
         p.hubbits[w] = 0ull;
       }
     }
@@ -1046,14 +1062,21 @@ __device__ __forceinline__ void flush_wave(const int2* surv, int cnt, int* ns, i
   }
 }
 
-// The listed vertices of one 256-slot chunk (one graph), by one wavefront;
-// `owner`, `list`, `lbase`, `lroot`, `lvtx` and `surv` are its LDS tables.
 template <bool BATCH>
-__device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
-                                                unsigned char* owner, unsigned char* list,
-                                                int* lbase, int* lroot, int* lvtx, int2* surv) {
+__global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
+  __shared__ unsigned char s_owner[4][kWaveWindow];
+  __shared__ unsigned char s_list[4][kSlotAlign];
+  __shared__ int s_base[4][64];
+  __shared__ int s_root[4][64];
+  __shared__ int s_vtx[4][64];
+  __shared__ int2 s_surv[4][kWaveSurvivors];
+  // one 256-slot chunk (one graph) per wavefront
+  const long long ws0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
+  if (ws0 >= p.nslots) return;
   const long long nwords = (p.nslots + 63) / 64;
   const int lane = lane_id();
+  const int wv = uniform((int)(threadIdx.x >> 6));
+  unsigned char* const list = s_list[wv];
   int cnt = 0;  // wave-uniform
 #pragma unroll
   for (int i = 0; i < kSlotAlign / 64; i++) {
@@ -1071,6 +1094,8 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
   const GraphRef G = resolve<BATCH>(p, ws0);
   int* const ns = G.nstat;
   const int* const nl = G.nlist;
+  unsigned char* const owner = s_owner[wv];
+  int2* const surv = s_surv[wv];
   const int R = p.filter ? p.giant[G.g] : -1;
   const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
   int nsurv = 0;  // wave-uniform
@@ -1096,9 +1121,9 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
     const int incl = (int)wave_incl_sum((unsigned)deg);
     const int off = incl - deg;
     const int total = __shfl(incl, 63);
-    lbase[lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
-    lroot[lane] = root;
-    lvtx[lane] = v;
+    s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
+    s_root[wv][lane] = root;
+    s_vtx[wv][lane] = v;
     // window positions are unsigned: the lists of one wavefront may hold
     // nearly 2^31 entries, and a step past the last must not wrap
     const unsigned utotal = (unsigned)total;
@@ -1122,7 +1147,7 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
           const unsigned e = e0 + k * 64 + lane;
           const bool valid = e < wend;
           src[k] = valid ? owner[e - w0] : 0;
-          u[k] = valid ? ld_stream(&nl[lbase[src[k]] + e]) : INT_MAX;
+          u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
         }
         unsigned keep = 0;
         if (skip) {
@@ -1131,7 +1156,7 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
           // owner is noted and hooked to R once, after its list
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            keep |= (u[k] != INT_MAX && u[k] != lvtx[src[k]]) ? (1u << k) : 0u;
+            keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) {
@@ -1141,7 +1166,7 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
           }
         } else {
 #pragma unroll
-          for (int k = 0; k < 4; k++) keep |= (lvtx[src[k]] > u[k]) ? (1u << k) : 0u;
+          for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
           if (mk) {
             unsigned in = 0;
 #pragma unroll
@@ -1169,7 +1194,7 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
           int at = nsurv + (int)(pos - c);
 #pragma unroll
           for (int k = 0; k < 4; k++)
-            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], lroot[src[k]]);
+            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
           nsurv += wtot;
         }
       }
@@ -1186,134 +1211,6 @@ __device__ __forceinline__ void hook_chunk_wave(const KParams& p, long long ws0,
     }
   }
   if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
-}
-
-// Filter 2: the lists of the high-degree vertices that mark listed (outside
-// the largest tree, or, in a graph without one, those that may have a
-// smaller neighbour), in tasks of hub_chunk entries.  Task t goes to
-// wavefront t mod (the wavefronts of the launch), after that wavefront's own
-// chunk, so a long list is spread over many wavefronts and no launch is
-// added for the usual case of no such vertex (one load of the counter per
-// wavefront).  A task streams its entries with coalesced loads, four per
-// lane per step, stages the ones that need a hook in LDS as (neighbour, root
-// estimate) pairs and hooks them together, as the chunk lists above do;
-// entries into the largest tree stand for one hook to its root.  (Dealt over
-// the lanes of one wavefront with the chunk lists instead, a vertex with a
-// million neighbours outside the largest tree took 10.3 ms.)
-template <bool BATCH>
-__device__ __forceinline__ void hook_hub_tasks(const KParams& p, int2* surv, long long gw,
-                                               long long nw) {
-  const unsigned long long ctr =
-      __hip_atomic_load(p.hubctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const long long pushed = (long long)(unsigned)ctr;
-  const long long nhubs = pushed < p.hub_cap ? pushed : p.hub_cap;
-  const long long ntasks = (long long)(ctr >> 32);
-  if (gw >= ntasks || nhubs <= 0) return;
-  const int lane = lane_id();
-  long long k = 0;  // hubs[k].first_task <= t throughout (hubs[0].first_task is 0)
-  for (long long t = gw; t < ntasks; t += nw) {
-    // the last hub whose first task is <= t: 64 probes per step; first
-    // tasks rise strictly with the index, every listed list holding a task
-    long long hi = nhubs;
-    while (hi - k > 1) {
-      const long long step = (hi - k + 63) / 64;
-      const long long probe = k + (long long)lane * step;
-      const bool ok = probe < hi && p.hubs[probe].first_task <= t;
-      const unsigned long long b = __ballot(ok);  // lanes 0..j, lane 0 always
-      const int j = 63 - __clzll((long long)b);
-      k += (long long)j * step;
-      hi = k + step < hi ? k + step : hi;
-    }
-    const HubEntry h = p.hubs[k];
-    const GraphRef G = resolve<BATCH>(p, h.slot);
-    int* const ns = G.nstat;
-    const int v = (int)(h.slot - G.slot0);
-    const int Rg = p.giant[G.g];
-    const long long cb = (long long)h.beg + (t - h.first_task) * (long long)p.hub_chunk;
-    const long long ce = cb + p.hub_chunk < (long long)h.end ? cb + p.hub_chunk : (long long)h.end;
-    int vroot = 0;
-    if (lane == 0) vroot = find_root(v, ns, p.path_mode);
-    vroot = __shfl(vroot, 0);
-    int nsurv = 0;  // wave-uniform
-    bool intree = false;
-    for (long long e0 = cb; e0 < ce; e0 += 256) {
-      int u[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const long long e = e0 + q * 64 + lane;
-        u[q] = e < ce ? ld_stream(&G.nlist[e]) : INT_MAX;
-      }
-      unsigned keep = 0;
-      if (Rg >= 0) {
-        // every entry; one into the largest tree is noted, not staged
-#pragma unroll
-        for (int q = 0; q < 4; q++) keep |= (u[q] != INT_MAX && u[q] != v) ? (1u << q) : 0u;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          if (((keep >> q) & 1u) && slot_bit(p.gbits, G.slot0 + u[q])) {
-            keep &= ~(1u << q);
-            intree = true;
-          }
-        }
-      } else {
-        // no largest tree: the entries below v, as the reference hooks them
-#pragma unroll
-        for (int q = 0; q < 4; q++) keep |= u[q] < v ? (1u << q) : 0u;
-      }
-      const unsigned c = __popc(keep);
-      const unsigned pos = wave_incl_sum(c);
-      const int wtot = (int)__shfl(pos, 63);
-      if (wtot) {
-        if (nsurv + wtot > kWaveSurvivors) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          flush_wave(surv, nsurv, ns, p.path_mode);
-          nsurv = 0;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-        int at = nsurv + (int)(pos - c);
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          if ((keep >> q) & 1u) surv[at++] = make_int2(u[q], vroot);
-        nsurv += wtot;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // all entries of the task into the largest tree join the same two trees
-    if (__ballot(intree) && lane == 0) (void)hook_edge(vroot, Rg, lab_ld(&ns[Rg]), ns, p.path_mode);
-  }
-}
-
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
-  __shared__ unsigned char s_owner[4][kWaveWindow];
-  __shared__ unsigned char s_list[4][kSlotAlign];
-  __shared__ int s_base[4][64];
-  __shared__ int s_root[4][64];
-  __shared__ int s_vtx[4][64];
-  __shared__ int2 s_surv[4][kWaveSurvivors];
-  const int wv = uniform((int)(threadIdx.x >> 6));
-  const long long gw = (long long)blockIdx.x * 4 + wv;
-  // one 256-slot chunk (one graph) per wavefront
-  const long long ws0 = gw * kSlotAlign;
-  if (ws0 < p.nslots)
-    hook_chunk_wave<BATCH>(p, ws0, s_owner[wv], s_list[wv], s_base[wv], s_root[wv], s_vtx[wv],
-                           s_surv[wv]);
-  if (p.filter == 2) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    hook_hub_tasks<BATCH>(p, s_surv[wv], gw, (long long)gridDim.x * 4);
-  }
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------
@@ -1354,9 +1251,17 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
   constexpr int CAP = 2 * R * BT;  // staged pairs; room for one more round when below R*BT
   __shared__ int2 stage[CAP];
   __shared__ int s_cnt;
-  const long long nsb = ((p.nslots + 63) / 64 + kScanBlock - 1) / kScanBlock;
-  const unsigned long long ctr = sum_block_totals(p, nsb);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *p.hubctr = ctr;  // for eclcc_workspace_stats
+  // the hubs and tasks: with filter 2 mark's counter (its atomic adds
+  // finished with an earlier kernel on this stream, so a plain load sees
+  // their sum), otherwise the scan's block totals
+  unsigned long long ctr;
+  if (p.filter == 2) {
+    ctr = *p.hubctr;
+  } else {
+    const long long nsb = ((p.nslots + 63) / 64 + kScanBlock - 1) / kScanBlock;
+    ctr = sum_block_totals(p, nsb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.hubctr = ctr;  // for eclcc_workspace_stats
+  }
   const long long pushed = (long long)(unsigned)ctr;
   const long long nhubs = pushed < p.hub_cap ? pushed : p.hub_cap;
   const long long ntasks = (long long)(ctr >> 32);
@@ -1513,6 +1418,10 @@ static unsigned spread_stride(unsigned blocks) {
   }
 }
 
+// Blocks a vertex-parallel launch should have at least (8 per CU), where
+// its per-block share of words can shrink.
+constexpr long long kMinBlocks = 2048;
+
 template <bool BATCH>
 static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s, hipEvent_t* ev) {
   const long long vb = (p.nslots + 255) / 256;
@@ -1530,8 +1439,12 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_init<BATCH><<<grid, block, 0, s>>>(p);
   mark_ev();
   if constexpr (BATCH) {
-    const unsigned nb = (unsigned)((words + kSampleWords - 1) / kSampleWords);
-    k_sample_dense<true><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
+    // words per sample block: kSampleWords, fewer where that leaves fewer
+    // than kMinBlocks blocks
+    int sspan = kSampleWords;
+    while (sspan > 1 && (words + sspan - 1) / sspan < kMinBlocks) sspan /= 2;
+    const unsigned nb = (unsigned)((words + sspan - 1) / sspan);
+    k_sample_dense<true><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb), sspan);
   } else {
     k_sample_lane<<<grid, block, 0, s>>>(p);
   }
@@ -1547,14 +1460,16 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   if (p.filter != 2) k_hub_scan<<<dim3((unsigned)sb), dim3(kScanBlock), 0, s>>>(p);
   mark_ev();
-  // words per compute1 block: as many as keep >= 2048 blocks, 4..kLowWords
+  // words per compute1 block: as many as keep >= kMinBlocks blocks, 4..kLowWords
   int span = 4;
-  while (span < kLowWords && words / (2 * span) >= 2048) span *= 2;
+  while (span < kLowWords && words / (2 * span) >= kMinBlocks) span *= 2;
   k_hook_thread<BATCH><<<dim3((unsigned)((words + span - 1) / span)), block, 0, s>>>(p, span);
   mark_ev();
   k_hook_wave<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
   mark_ev();
-  if (p.filter != 2) {
+  {
+    // filter 2 too: the lists of the high-degree vertices mark left listed
+    // (few or none; every block then leaves after one load)
     const dim3 hgrid((unsigned)shape.hub_blocks);
     switch (shape.block_threads) {
       case 1024: k_hook_block<BATCH, 1024><<<hgrid, dim3(1024), 0, s>>>(p); break;

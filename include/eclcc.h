@@ -176,7 +176,9 @@ int eclcc_run_init_only(int32_t nodes, int32_t edges, const int32_t* d_nindex,
                         void* stream);
 
 /* Reads the worklist counters of the last run in d_work (synchronises
-   `stream`): number of high-degree vertices and of high-degree edge tasks. */
+   `stream`): number of high-degree vertices whose lists that run hooked in
+   edge tasks (with giant_filter 2 only those outside the sampled largest
+   tree), and number of those tasks. */
 int eclcc_workspace_stats(const void* d_work, size_t work_bytes, void* stream, int64_t* n_hubs,
                           int64_t* n_hub_tasks);
 

@@ -275,7 +275,7 @@ def test_high_degree_vertices_outside_the_largest_tree(cuda, filt):
 def test_large_stars_outside_the_largest_tree(cuda, kw):
     # three stars of 50,000 leaves beside a random graph of 200,000 vertices:
     # with filter 2 mark lists the three centres (outside the largest tree)
-    # with their edge tasks, and compute2's wavefronts hook their lists in
+    # with their edge tasks, and compute3's blocks hook their lists in
     # tasks of hub_chunk entries
     from eclcc import params_for_graph
 
