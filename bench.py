@@ -51,6 +51,11 @@ def make_workload(name: str):
         return workloads.er65536(1)
     if name.startswith("rmat"):
         return workloads.rmat(int(name[4:]))
+    if name.startswith("erstars-"):
+        # erstars-<stars>x<leaves>: stars beside a random graph of 4M vertices
+        # and 16M pairs (seed 7), their centres outside its largest tree
+        stars, leaves = (int(x) for x in name[len("erstars-"):].split("x"))
+        return workloads.er_with_stars(4_000_000, 16_000_000, stars, leaves, 7)
     raise SystemExit(f"unknown workload {name}")
 
 

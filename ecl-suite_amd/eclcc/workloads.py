@@ -4,6 +4,8 @@
     rmat24      R-MAT scale 24, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05),
                 symmetrised, deduplicated, self-loops dropped, IDs permuted
     grid256     3-D 7-point grid 256^3, each edge kept with p = 0.6, IDs permuted
+    er_with_stars  a random graph beside stars whose centres lie outside its
+                largest tree (numpy's generator)
     batch_mix   graphs with n log-uniform on [1e3, 1e6]; 50/50 Erdos-Renyi
                 (average degree uniform on [4, 32]) / R-MAT (edge factor 8)
 
@@ -48,6 +50,28 @@ def grid3d(side: int, keep_p: float = 0.6, seed: int = 3) -> Graph:
 
 def grid256(seed: int = 3) -> Graph:
     return grid3d(256, 0.6, seed)
+
+
+def er_with_stars(n_er: int, pairs: int, stars: int, leaves: int, seed: int) -> Graph:
+    """A random graph on n_er vertices (`pairs` random pairs) beside `stars`
+    stars of `leaves` leaves each, all IDs permuted (numpy's seeded
+    generator).  The random part holds most vertices, so its tree is the
+    largest, and every star centre is a high-degree vertex outside it: the
+    case where the default filter leaves long lists to the hub kernel."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    src = [rng.integers(0, n_er, pairs)]
+    dst = [rng.integers(0, n_er, pairs)]
+    base = n_er
+    for _ in range(stars):
+        src.append(np.full(leaves, base, dtype=np.int64))
+        dst.append(np.arange(base + 1, base + 1 + leaves, dtype=np.int64))
+        base += 1 + leaves
+    perm = rng.permutation(base)
+    g = Graph.from_edges(base, perm[np.concatenate(src)], perm[np.concatenate(dst)])
+    g.name = f"er{n_er}+{stars}stars{leaves}-seed{seed}"
+    return g
 
 
 @dataclass

@@ -73,22 +73,11 @@ def two_stars_bridged(n_leaves, hub_a, hub_b, n):
 
 
 def er_with_stars(n_er, pairs, stars, leaves, seed):
-    """A random graph on n_er vertices (`pairs` seeded random pairs), beside
-    `stars` stars of `leaves` leaves each, all IDs permuted.  The random part
-    holds most vertices, so its tree is the largest, and every star centre is
-    a high-degree vertex outside it."""
-    rng = np.random.default_rng(seed)
-    src = [rng.integers(0, n_er, pairs)]
-    dst = [rng.integers(0, n_er, pairs)]
-    base = n_er
-    for _ in range(stars):
-        src.append(np.full(leaves, base, dtype=np.int64))
-        dst.append(np.arange(base + 1, base + 1 + leaves, dtype=np.int64))
-        base += 1 + leaves
-    perm = rng.permutation(base)
-    s = perm[np.concatenate(src)]
-    d = perm[np.concatenate(dst)]
-    return from_undirected(base, np.stack([s, d], axis=1), f"er{n_er}+{stars}stars{leaves}")
+    """Stars whose centres lie outside the largest tree (the package's
+    generator, also a bench workload)."""
+    from eclcc import workloads
+
+    return workloads.er_with_stars(n_er, pairs, stars, leaves, seed)
 
 
 INT32_MAX = 2**31 - 1
