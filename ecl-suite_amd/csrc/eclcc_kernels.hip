@@ -1372,36 +1372,39 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
 
 // ---- flatten (ECL-CC-atomic.cu:250-264) ---------------------------------------
 
-// A wavefront takes one 256-slot chunk (one graph): four vertices per lane,
-// 64 slots apart, their walks in flight together.  The walks are short
-// (mark and compute leave most labels one step from the root), so the pass
-// is bound by load latency and four independent chains per lane hide it
-// (R-MAT-24: 0.033 -> 0.024 ms; grid 256^3: 0.027 -> 0.019 ms).
+// A wavefront takes two 256-slot chunks (each of one graph, possibly two
+// different ones): eight vertices per lane, 64 slots apart, their walks in
+// flight together.  The walks are short (mark and compute leave most labels
+// one step from the root), so the pass is bound by load latency and
+// independent chains per lane hide it (R-MAT-24: 0.033 -> 0.024 ms at four
+// per lane; grid 256^3: 0.027 -> 0.019 ms; eight per lane: the 4096-graph
+// mix 0.81 -> 0.65 ms, one large graph unchanged).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_flatten(KParams p) {
-  constexpr int K = kSlotAlign / 64;
-  const long long ws = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
+  constexpr int K = 2 * kSlotAlign / 64;  // two chunks per wavefront
+  const long long ws = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * (2 * kSlotAlign);
   if (ws >= p.nslots) return;
-  const GraphRef G = resolve<BATCH>(p, ws);
-  int* const ns = G.nstat;
+  const GraphRef G0 = resolve<BATCH>(p, ws);
+  const GraphRef G1 = ws + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws + kSlotAlign) : G0;
   const int lane = lane_id();
   int v[K], s[K], old[K];
   bool go[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
+    const GraphRef& G = k < K / 2 ? G0 : G1;
     const long long slot = ws + k * 64 + lane;
     v[k] = (int)(slot - G.slot0);
     go[k] = slot < p.nslots && v[k] < G.n;
   }
 #pragma unroll
-  for (int k = 0; k < K; k++) s[k] = go[k] ? lab_ld(&ns[v[k]]) : 0;
+  for (int k = 0; k < K; k++) s[k] = go[k] ? lab_ld(&(k < K / 2 ? G0 : G1).nstat[v[k]]) : 0;
 #pragma unroll
   for (int k = 0; k < K; k++) old[k] = s[k];
   bool more = true;
   while (more) {
     int nx[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) nx[k] = (go[k] && s[k] != 0) ? lab_ld(&ns[s[k]]) : s[k];
+    for (int k = 0; k < K; k++) nx[k] = (go[k] && s[k] != 0) ? lab_ld(&(k < K / 2 ? G0 : G1).nstat[s[k]]) : s[k];
     more = false;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -1415,8 +1418,9 @@ __global__ __launch_bounds__(256) void k_flatten(KParams p) {
   }
 #pragma unroll
   for (int k = 0; k < K; k++) {
+    const GraphRef& G = k < K / 2 ? G0 : G1;
     const long long slot = ws + k * 64 + lane;
-    if (slot < p.nslots && v[k] < G.n && s[k] != old[k]) lab_st(&ns[v[k]], s[k]);
+    if (slot < p.nslots && v[k] < G.n && s[k] != old[k]) lab_st(&G.nstat[v[k]], s[k]);
   }
 }
 
@@ -1497,7 +1501,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
     }
   }
   mark_ev();
-  k_flatten<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
+  k_flatten<BATCH><<<dim3((unsigned)((p.nslots + 8 * kSlotAlign - 1) / (8 * kSlotAlign))), block, 0, s>>>(p);
   mark_ev();
   return hipGetLastError();
 }
