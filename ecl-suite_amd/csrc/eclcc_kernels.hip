@@ -941,24 +941,29 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   }
 }
 
-// Batches: a wavefront takes one 256-slot chunk (one graph), four vertices
-// per lane, their walks toward R in flight together (the walks are short and
-// bound by load latency, as in flatten; 1024-graph batch: 1.71 -> 1.52 ms).
+// Batches: a wavefront takes two 256-slot chunks (each of one graph, possibly
+// two different ones), eight vertices per lane, their walks toward R in
+// flight together (the walks are bound by load latency, as in flatten;
+// 1024-graph batch: 1.71 -> 1.52 ms at four per lane; 4096-graph mix: 6.14
+// -> 5.75 ms at eight).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
-  constexpr int K = kSlotAlign / 64;
+  constexpr int K = 2 * kSlotAlign / 64;  // two chunks per wavefront
+  constexpr int H = K / 2;
   const long long ws0 =
-      ((long long)spread_block(stride) * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
+      ((long long)spread_block(stride) * 4 + uniform((int)(threadIdx.x >> 6))) * (2 * kSlotAlign);
   if (ws0 >= p.nslots) return;
   const long long nwords = (p.nslots + 63) / 64;
-  const GraphRef G = resolve<BATCH>(p, ws0);
-  int* const ns = G.nstat;
+  const GraphRef G0 = resolve<BATCH>(p, ws0);
+  const GraphRef G1 = ws0 + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws0 + kSlotAlign) : G0;
   const int lane = lane_id();
-  const int R = p.filter ? p.giant[G.g] : -1;
+  const int R0 = p.filter ? p.giant[G0.g] : -1;
+  const int R1 = p.filter ? p.giant[G1.g] : -1;
   int v[K], first[K], cur[K], prev[K];
   bool live[K], go[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
+    const GraphRef& G = k < H ? G0 : G1;
     const long long slot = ws0 + k * 64 + lane;
     v[k] = (int)(slot - G.slot0);
     live[k] = slot < p.nslots && v[k] < G.n;
@@ -976,12 +981,13 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
     hubw[k] = in ? p.hubbits[w] : 0ull;
   }
 #pragma unroll
-  for (int k = 0; k < K; k++) first[k] = (R >= 0 && live[k]) ? lab_ld(&ns[v[k]]) : 0;
+  for (int k = 0; k < K; k++)
+    first[k] = ((k < H ? R0 : R1) >= 0 && live[k]) ? lab_ld(&(k < H ? G0 : G1).nstat[v[k]]) : 0;
 #pragma unroll
   for (int k = 0; k < K; k++) {
     cur[k] = first[k];
     prev[k] = v[k];
-    go[k] = R >= 0 && live[k] && first[k] != v[k];
+    go[k] = (k < H ? R0 : R1) >= 0 && live[k] && first[k] != v[k];
   }
   // find_until(v, first, R) for the K vertices at once
   bool more = true;
@@ -989,12 +995,14 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
     int nx[K];
 #pragma unroll
     for (int k = 0; k < K; k++)
-      nx[k] = (go[k] && cur[k] != R && cur[k] != 0) ? lab_ld(&ns[cur[k]]) : cur[k];
+      nx[k] = (go[k] && cur[k] != (k < H ? R0 : R1) && cur[k] != 0)
+                  ? lab_ld(&(k < H ? G0 : G1).nstat[cur[k]])
+                  : cur[k];
     more = false;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      if (go[k] && cur[k] != R && cur[k] != 0 && cur[k] > nx[k]) {
-        if (p.path_mode == 0) lab_st(&ns[prev[k]], nx[k]);
+      if (go[k] && cur[k] != (k < H ? R0 : R1) && cur[k] != 0 && cur[k] > nx[k]) {
+        if (p.path_mode == 0) lab_st(&(k < H ? G0 : G1).nstat[prev[k]], nx[k]);
         prev[k] = cur[k];
         cur[k] = nx[k];
         more = true;
@@ -1005,12 +1013,14 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
   }
 #pragma unroll
   for (int k = 0; k < K; k++)
-    if (p.path_mode != 0 && R >= 0 && live[k] && first[k] != v[k] && cur[k] != first[k])
-      lab_st(&ns[v[k]], cur[k]);
+    if (p.path_mode != 0 && (k < H ? R0 : R1) >= 0 && live[k] && first[k] != v[k] && cur[k] != first[k])
+      lab_st(&(k < H ? G0 : G1).nstat[v[k]], cur[k]);
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const long long w = (ws0 >> 6) + k;
     if (w >= nwords) break;
+    const GraphRef& G = k < H ? G0 : G1;
+    const int R = k < H ? R0 : R1;
     const bool member = R >= 0 && live[k] && (v[k] == R || (first[k] != v[k] && cur[k] == R));
     const unsigned long long gw = __ballot(member);
     unsigned long long keep = ~0ull;
@@ -1475,7 +1485,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
   mark_ev();
   if constexpr (BATCH) {
-    const unsigned nb = (unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign));
+    const unsigned nb = (unsigned)((p.nslots + 8 * kSlotAlign - 1) / (8 * kSlotAlign));
     k_mark<true><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
   } else {
     k_mark_lane<<<grid, block, 0, s>>>(p);
