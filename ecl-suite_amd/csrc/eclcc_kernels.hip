@@ -543,34 +543,47 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
 // faster (R-MAT-24: 0.20 vs 0.23 ms; 3-D grid 256^3: 0.67 vs 0.73 ms): a
 // dense thread walks its vertices one after the other, and the last blocks
 // finish late.  Blocks of fewer words are slower on batches (1024 graphs,
-// sample with 8 / 16 / 32 words per block: +90 / +55 / +20 %): the blocks
-// that run together then cover fewer graphs (see spread_block).  A small
-// batch (the server's, of a few dozen graphs) takes fewer words per block,
-// `span`, so that the launch still has enough blocks to fill the GPU: at 64
-// words a batch of one ER graph with 65,536 vertices has 16 blocks.
-constexpr int kSampleWords = 64;
+// sample with 8 / 16 / 32 words per block: +90 / +55 / +20 % against 64):
+// the blocks that run together then cover fewer graphs (see spread_block).
+// 128 words per block, their list prefix taken over two wavefronts, gained
+// again (4096-graph mix: 8.01 -> 7.53 ms).  A small batch (the server's, of
+// a few dozen graphs) takes fewer words per block, `span`, so that the
+// launch still has enough blocks to fill the GPU: at 64 words a batch of
+// one ER graph with 65,536 vertices had 16 blocks.
+constexpr int kSampleWords = 128;
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride, int span) {
-  static_assert(kSampleWords <= 64, "one word per lane of the first wavefront");
+  static_assert(kSampleWords <= 128, "one word per lane of the first two wavefronts");
   constexpr int kChunks = kSampleWords * 64 / kSlotAlign;
   __shared__ unsigned short s_list[kSampleWords * 64];
   __shared__ unsigned long long s_bits[kSampleWords];
   __shared__ int s_off[kSampleWords];
-  __shared__ int s_cnt;
+  __shared__ unsigned s_wtot[2];
   __shared__ int* s_ns[kChunks];
   __shared__ long long s_s0[kChunks];
   const long long words = (p.nslots + 63) / 64;
   const long long w0 = (long long)spread_block(stride) * span;
-  if (threadIdx.x < 64) {
-    (void)gather_words(p.smpbits, w0, span, words, s_bits, s_off, &s_cnt);
-  } else if (threadIdx.x < 64 + kChunks) {
+  const int tid = (int)threadIdx.x;
+  if (tid < 128) {
+    // the block's words, one per thread of the first two wavefronts, and
+    // each word's place in the block's list (a prefix sum of their counts)
+    const long long w = w0 + tid;
+    const unsigned long long b = (tid < span && w < words) ? p.smpbits[w] : 0ull;
+    const unsigned c = (unsigned)__popcll(b);
+    const unsigned incl = wave_incl_sum(c);
+    if (tid < span) {
+      s_bits[tid] = b;
+      s_off[tid] = (int)(incl - c);
+    }
+    if (lane_id() == 63) s_wtot[tid >> 6] = incl;
+  } else if (tid < 128 + kChunks) {
     // the graph of each 256-slot chunk of the block, resolved once here
     // instead of per vertex (two dependent loads: chunk map, then the graph
     // table; 4096-graph mix: 8.04 -> 8.01 ms, the loads mostly hit L2).
     // The block's slots start at a chunk boundary when span >= 4; a
     // smaller span keeps the block inside one chunk (c = 0 below).
-    const int c = (int)threadIdx.x - 64;
+    const int c = tid - 128;
     const long long slot = (w0 << 6) + (long long)c * kSlotAlign;
     if (c * kSlotAlign < span * 64 && slot < p.nslots) {
       const GraphRef G = resolve<BATCH>(p, slot);
@@ -579,10 +592,12 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride
     }
   }
   __syncthreads();
+  if (tid >= 64 && tid < 128 && tid < span) s_off[tid] += (int)s_wtot[0];
+  const int cnt = (int)(s_wtot[0] + (span > 64 ? s_wtot[1] : 0u));
+  __syncthreads();
   list_words(s_bits, s_off, span, s_list);
   __syncthreads();
-  const int cnt = s_cnt;
-  for (int j = threadIdx.x; j < cnt; j += 256) {
+  for (int j = tid; j < cnt; j += 256) {
     const int e = s_list[j];
     const long long slot = (w0 << 6) + e;
     const int c = e / kSlotAlign;
@@ -941,24 +956,25 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
   }
 }
 
-// Batches: a wavefront takes two 256-slot chunks (each of one graph, possibly
-// two different ones), eight vertices per lane, their walks toward R in
+// Batches: a wavefront takes CPW 256-slot chunks (each of one graph, possibly
+// two different ones), 4 * CPW vertices per lane, their walks toward R in
 // flight together (the walks are bound by load latency, as in flatten;
 // 1024-graph batch: 1.71 -> 1.52 ms at four per lane; 4096-graph mix: 6.14
-// -> 5.75 ms at eight).
-template <bool BATCH>
+// -> 5.75 ms at eight).  The launch takes CPW = 2 for large batches only.
+template <bool BATCH, int CPW>
 __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
-  constexpr int K = 2 * kSlotAlign / 64;  // two chunks per wavefront
-  constexpr int H = K / 2;
+  constexpr int H = kSlotAlign / 64;  // words per chunk
+  constexpr int K = CPW * H;          // CPW chunks per wavefront
   const long long ws0 =
-      ((long long)spread_block(stride) * 4 + uniform((int)(threadIdx.x >> 6))) * (2 * kSlotAlign);
+      ((long long)spread_block(stride) * 4 + uniform((int)(threadIdx.x >> 6))) * (CPW * kSlotAlign);
   if (ws0 >= p.nslots) return;
   const long long nwords = (p.nslots + 63) / 64;
   const GraphRef G0 = resolve<BATCH>(p, ws0);
-  const GraphRef G1 = ws0 + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws0 + kSlotAlign) : G0;
+  const GraphRef G1 =
+      CPW == 2 && ws0 + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws0 + kSlotAlign) : G0;
   const int lane = lane_id();
   const int R0 = p.filter ? p.giant[G0.g] : -1;
-  const int R1 = p.filter ? p.giant[G1.g] : -1;
+  const int R1 = CPW == 2 && p.filter ? p.giant[G1.g] : R0;
   int v[K], first[K], cur[K], prev[K];
   bool live[K], go[K];
 #pragma unroll
@@ -1382,39 +1398,40 @@ __global__ __launch_bounds__(BT) void k_hook_block(KParams p) {
 
 // ---- flatten (ECL-CC-atomic.cu:250-264) ---------------------------------------
 
-// A wavefront takes two 256-slot chunks (each of one graph, possibly two
-// different ones): eight vertices per lane, 64 slots apart, their walks in
-// flight together.  The walks are short (mark and compute leave most labels
+// A wavefront takes CPW 256-slot chunks (each of one graph, possibly two
+// different ones): 4 * CPW vertices per lane, 64 slots apart, their walks in
+// flight together; the launch takes CPW = 2 for large launches only.  The walks are short (mark and compute leave most labels
 // one step from the root), so the pass is bound by load latency and
 // independent chains per lane hide it (R-MAT-24: 0.033 -> 0.024 ms at four
 // per lane; grid 256^3: 0.027 -> 0.019 ms; eight per lane: the 4096-graph
 // mix 0.81 -> 0.65 ms, one large graph unchanged).
-template <bool BATCH>
+template <bool BATCH, int CPW>
 __global__ __launch_bounds__(256) void k_flatten(KParams p) {
-  constexpr int K = 2 * kSlotAlign / 64;  // two chunks per wavefront
-  const long long ws = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * (2 * kSlotAlign);
+  constexpr int H = kSlotAlign / 64;  // words per chunk
+  constexpr int K = CPW * H;          // CPW chunks per wavefront
+  const long long ws = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * (CPW * kSlotAlign);
   if (ws >= p.nslots) return;
   const GraphRef G0 = resolve<BATCH>(p, ws);
-  const GraphRef G1 = ws + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws + kSlotAlign) : G0;
+  const GraphRef G1 = CPW == 2 && ws + kSlotAlign < p.nslots ? resolve<BATCH>(p, ws + kSlotAlign) : G0;
   const int lane = lane_id();
   int v[K], s[K], old[K];
   bool go[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const GraphRef& G = k < K / 2 ? G0 : G1;
+    const GraphRef& G = k < H ? G0 : G1;
     const long long slot = ws + k * 64 + lane;
     v[k] = (int)(slot - G.slot0);
     go[k] = slot < p.nslots && v[k] < G.n;
   }
 #pragma unroll
-  for (int k = 0; k < K; k++) s[k] = go[k] ? lab_ld(&(k < K / 2 ? G0 : G1).nstat[v[k]]) : 0;
+  for (int k = 0; k < K; k++) s[k] = go[k] ? lab_ld(&(k < H ? G0 : G1).nstat[v[k]]) : 0;
 #pragma unroll
   for (int k = 0; k < K; k++) old[k] = s[k];
   bool more = true;
   while (more) {
     int nx[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) nx[k] = (go[k] && s[k] != 0) ? lab_ld(&(k < K / 2 ? G0 : G1).nstat[s[k]]) : s[k];
+    for (int k = 0; k < K; k++) nx[k] = (go[k] && s[k] != 0) ? lab_ld(&(k < H ? G0 : G1).nstat[s[k]]) : s[k];
     more = false;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -1428,7 +1445,7 @@ __global__ __launch_bounds__(256) void k_flatten(KParams p) {
   }
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const GraphRef& G = k < K / 2 ? G0 : G1;
+    const GraphRef& G = k < H ? G0 : G1;
     const long long slot = ws + k * 64 + lane;
     if (slot < p.nslots && v[k] < G.n && s[k] != old[k]) lab_st(&G.nstat[v[k]], s[k]);
   }
@@ -1454,6 +1471,9 @@ static unsigned spread_stride(unsigned blocks) {
 // Blocks a vertex-parallel launch should have at least (8 per CU), where
 // its per-block share of words can shrink.
 constexpr long long kMinBlocks = 2048;
+// Wavefronts a two-chunk launch must have at least: two rounds of the GPU's
+// 8192 resident wavefronts (256 CUs x 4 SIMDs x 8).
+constexpr long long kTwoChunkWaves = 16384;
 
 template <bool BATCH>
 static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s, hipEvent_t* ev) {
@@ -1463,6 +1483,11 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   const long long words = (p.nslots + 63) / 64;
   const long long sb = (words + kScanBlock - 1) / kScanBlock;
   const dim3 grid((unsigned)vb), block(256);
+  // chunks per wavefront of the batched mark and of flatten: two (eight
+  // walks per lane) where that still leaves two rounds of resident
+  // wavefronts, else one (a small batch would leave CUs idle: a one-graph
+  // batch of ER-65,536 took 86.7 against 72.6 us)
+  const int cpw = p.nslots / (2 * kSlotAlign) >= kTwoChunkWaves ? 2 : 1;
   int k = 0;
   auto mark_ev = [&]() {
     if (ev) (void)hipEventRecord(ev[k], s);
@@ -1485,8 +1510,9 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   k_pick<BATCH><<<dim3((unsigned)p.ngraphs), dim3(kPickSamples), 0, s>>>(p);
   mark_ev();
   if constexpr (BATCH) {
-    const unsigned nb = (unsigned)((p.nslots + 8 * kSlotAlign - 1) / (8 * kSlotAlign));
-    k_mark<true><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
+    const unsigned nb = (unsigned)((p.nslots + 4 * cpw * kSlotAlign - 1) / (4 * cpw * kSlotAlign));
+    if (cpw == 2) k_mark<true, 2><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
+    else k_mark<true, 1><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
   } else {
     k_mark_lane<<<grid, block, 0, s>>>(p);
   }
@@ -1511,7 +1537,11 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
     }
   }
   mark_ev();
-  k_flatten<BATCH><<<dim3((unsigned)((p.nslots + 8 * kSlotAlign - 1) / (8 * kSlotAlign))), block, 0, s>>>(p);
+  {
+    const dim3 fgrid((unsigned)((p.nslots + 4 * cpw * kSlotAlign - 1) / (4 * cpw * kSlotAlign)));
+    if (cpw == 2) k_flatten<BATCH, 2><<<fgrid, block, 0, s>>>(p);
+    else k_flatten<BATCH, 1><<<fgrid, block, 0, s>>>(p);
+  }
   mark_ev();
   return hipGetLastError();
 }
