@@ -70,9 +70,9 @@ struct KParams {
   unsigned long long* gbits;     // members of the sampled largest component (mark -> compute2/3)
   unsigned long long* blocktot;  // per kScanBlock words: totals, then exclusive prefix
   int* giant;                    // per graph: sampled root of the largest component, or -1
-  int* second;                   // the entry sample hooks (init -> sample): one graph, per
-                                 // slot; batches, per word, the sampled slots' entries
-                                 // packed at its front by rank
+  int* second;                   // the entry sample hooks (init -> sample): per 64-slot
+                                 // word, the sampled slots' entries packed at its front
+                                 // by rank
   HubEntry* hubs;                // high-degree vertices with their first task number
   unsigned long long* hubctr;    // low 32 bits: hubs; high 32: edge tasks
   unsigned int* flags;           // bit 0: hub list overflow

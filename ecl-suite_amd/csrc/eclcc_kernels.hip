@@ -460,10 +460,7 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
                      ((unsigned)v * 0x9E3779B1u) >> (32 - p.sample_shift) == 0u) &&
                     end - beg >= p.sample_min_deg;
   const bool sampled = active ? (x1 < v && pick) : (live && x0 != INT_MAX && x0 != v);
-  if (live) {
-    G.nstat[v] = m;  // no other thread reads labels during init
-    if constexpr (!BATCH) p.second[slot] = hooked;
-  }
+  if (live) G.nstat[v] = m;  // no other thread reads labels during init
   // degree bins: of the vertices that have a smaller neighbour (they hook
   // their entries below them), and with filter 2 of every vertex that has
   // a neighbour (mark then keeps those outside the largest tree)
@@ -479,13 +476,11 @@ __global__ __launch_bounds__(256) void k_init(KParams p) {
   const bool maybe = !full && live && m == v && pos < uend;
   const unsigned long long actmask = __ballot(active || maybe);
   const unsigned long long smpmask = __ballot(sampled);
-  // batches: the sampled entries of the word packed at its front, by rank
-  // (sample reads them from its list): about a quarter of the bytes
-  // written and read on the mix (4096 graphs: 23.97 -> 23.68 ms); one
-  // large graph's lane-per-vertex sample loads its entry before it knows
-  // whether it samples, so it keeps one per slot
-  if constexpr (BATCH)
-    if (sampled) p.second[ws + lanes_below(smpmask)] = hooked;
+  // the sampled vertices' entries, packed at the front of the word's 64
+  // entries by rank: about a quarter of the bytes written here and read by
+  // sample (4096-graph mix: 23.97 -> 23.68 ms; R-MAT-24 init -3 us, grid
+  // 256^3 0.932 -> 0.927 ms)
+  if (sampled) p.second[ws + lanes_below(smpmask)] = hooked;
   const unsigned long long lowmask = __ballot(low);
   const unsigned long long midmask = __ballot(mid);
   const unsigned long long hubmask = __ballot(hub);
@@ -528,14 +523,15 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
   const int lane = lane_id();
   int* const ns = p.nstat;
   const int v = (int)(ws + lane);
-  // the lane's entry and label are loaded together with the word that says
-  // whether it hooks at all (one latency less; the lines are read anyway
-  // whenever a lane of the wavefront hooks)
+  // the lane's label is loaded together with the word that says whether it
+  // hooks at all (one latency less; the lines are read anyway whenever a
+  // lane of the wavefront hooks); its entry sits at its rank among the
+  // word's sampled vertices
   const int vi = v < p.n ? v : 0;  // lanes past the end load vertex 0's
-  const int u = p.second[vi];
   const int s0 = lab_ld(&ns[vi]);
   const unsigned long long act = p.smpbits[ws >> 6];
   if (!((act >> lane) & 1ull)) return;
+  const int u = p.second[ws + lanes_below(act)];
   const int su = lab_ld(&ns[u]);
   const int top = p.path_mode == 2 ? meet_link_halve(s0, su, ns) : meet_link(s0, su, ns);
   if (top < s0 && s0 != v) lab_st(&ns[v], top);
