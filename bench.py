@@ -539,7 +539,9 @@ def gather_probe(args) -> dict:
     walks: ecl-suite_amd/bin/gather_probe gathers 2^26 uniformly random
     entries of a 64 MiB table (the label array of a 2^24-vertex graph), one
     independent gather per lane, with plain, relaxed agent-scope (the
-    library's label loads) and non-temporal loads, then copies 256 MiB.
+    library's label loads) and non-temporal loads, then a relaxed
+    agent-scope CAS (the hooks' flavour) and atomic add on the same
+    addresses, then copies 256 MiB.
     Under rocprofv3 --pmc TCC_EA0_RDREQ_* it gives each flavour's request
     sizes; here its JSON lines are gathered into one."""
     import subprocess
