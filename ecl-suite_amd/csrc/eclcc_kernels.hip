@@ -1072,7 +1072,9 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
 }
 
 // ---- compute2 (ECL-CC-atomic.cu:161-199) --------------------------------------
-// Wavefront w owns the 256-slot chunk w (one graph), gathers its medium-
+// Wavefront w owns GC consecutive 256-slot chunks (each of one graph; GC is
+// 1, 4 or 16, by launch size) and reads their medium-degree bits in one
+// load.  For each chunk that lists a vertex it gathers the chunk's medium-
 // degree vertices into an LDS list, and processes them in groups of up to
 // 64, one per lane: their adjacency lists are laid end to end and dealt out
 // evenly over the 64 lanes, four entries per lane per step, so a wavefront
@@ -1086,7 +1088,8 @@ __global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
 // here lie outside the largest tree and every entry is kept, a neighbour
 // inside the tree replaced by its root.  The kept entries are staged in LDS
 // as (neighbour, root estimate) pairs and hooked together when the stage
-// fills and at the end, so the streaming steps never wait for label chains.
+// fills and at the end of each chunk, so the streaming steps never wait for
+// label chains.
 
 constexpr int kWaveWindow = 2048;
 constexpr int kWaveSurvivors = 256;  // staged (neighbour, root estimate) pairs per wavefront
@@ -1111,155 +1114,166 @@ __device__ __forceinline__ void flush_wave(const int2* surv, int cnt, int* ns, i
   }
 }
 
-template <bool BATCH>
+template <bool BATCH, int GC>
 __global__ __launch_bounds__(256) void k_hook_wave(KParams p) {
+  static_assert(GC >= 1 && GC * (kSlotAlign / 64) <= 64, "one medium word per lane");
+  constexpr int H = kSlotAlign / 64;  // medium words per chunk
   __shared__ unsigned char s_owner[4][kWaveWindow];
   __shared__ unsigned char s_list[4][kSlotAlign];
   __shared__ int s_base[4][64];
   __shared__ int s_root[4][64];
   __shared__ int s_vtx[4][64];
   __shared__ int2 s_surv[4][kWaveSurvivors];
-  // one 256-slot chunk (one graph) per wavefront
-  const long long ws0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * kSlotAlign;
-  if (ws0 >= p.nslots) return;
+  // GC consecutive 256-slot chunks (each of one graph) per wavefront; their
+  // medium words come in one load, one word per lane, and a chunk that lists
+  // no vertex costs nothing more
+  const long long c0 = ((long long)blockIdx.x * 4 + uniform((int)(threadIdx.x >> 6))) * GC;
+  if (c0 * kSlotAlign >= p.nslots) return;
   const long long nwords = (p.nslots + 63) / 64;
   const int lane = lane_id();
   const int wv = uniform((int)(threadIdx.x >> 6));
+  const long long wl = c0 * H + lane;
+  const unsigned long long mword = lane < GC * H && wl < nwords ? p.midbits[wl] : 0ull;
+  const unsigned long long busy = __ballot(mword != 0ull);
   unsigned char* const list = s_list[wv];
-  int cnt = 0;  // wave-uniform
+  for (int c = 0; c < GC; c++) {
+    if (((busy >> (c * H)) & ((1ull << H) - 1ull)) == 0ull) continue;
+    const long long ws0 = (c0 + c) * kSlotAlign;
+    int cnt = 0;  // wave-uniform
 #pragma unroll
-  for (int i = 0; i < kSlotAlign / 64; i++) {
-    const long long w = (ws0 >> 6) + i;
-    const unsigned long long wbits = w < nwords ? p.midbits[w] : 0ull;
-    const unsigned long long bits =
-        ((unsigned long long)uniform((int)(wbits >> 32)) << 32) | (unsigned)uniform((int)wbits);
-    if ((bits >> lane) & 1ull) list[cnt + lanes_below(bits)] = (unsigned char)(i * 64 + lane);
-    cnt += __popcll(bits);
-  }
-  if (!cnt) return;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const GraphRef G = resolve<BATCH>(p, ws0);
-  int* const ns = G.nstat;
-  const int* const nl = G.nlist;
-  unsigned char* const owner = s_owner[wv];
-  int2* const surv = s_surv[wv];
-  const int R = p.filter ? p.giant[G.g] : -1;
-  const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
-  int nsurv = 0;  // wave-uniform
-  unsigned long long intree = 0;  // per lane: owners seen with an entry inside R's tree
-  // groups of up to 64 listed vertices, one per lane
-  for (int g0 = 0; g0 < cnt; g0 += 64) {
-    const int gn = cnt - g0 < 64 ? cnt - g0 : 64;
-    const unsigned long long bits = gn == 64 ? ~0ull : ((1ull << gn) - 1ull);
-    const bool mine = lane < gn;
-    const long long slot = ws0 + (mine ? list[g0 + lane] : 0);
-    const int v = (int)(slot - G.slot0);
-    // a marked vertex is joined to the sampled root: that root is a valid
-    // estimate (the CAS of a hook checks it), no walk needed
-    const bool marked = mine && p.filter && !skip && slot_bit(p.gbits, slot);
-    const unsigned long long mk = __ballot(marked);
-    int beg = 0, deg = 0, root = 0;
-    if (mine) {
-      root = marked ? R : lab_ld(&ns[v]);
-      beg = G.nidx[v];
-      deg = G.nidx[v + 1] - beg;
+    for (int i = 0; i < H; i++) {
+      const unsigned long long bits = __shfl(mword, c * H + i);
+      if ((bits >> lane) & 1ull) list[cnt + lanes_below(bits)] = (unsigned char)(i * 64 + lane);
+      cnt += __popcll(bits);
     }
-    if (mine && !marked) root = find_from(v, root, ns, p.path_mode);
-    const int incl = (int)wave_incl_sum((unsigned)deg);
-    const int off = incl - deg;
-    const int total = __shfl(incl, 63);
-    s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
-    s_root[wv][lane] = root;
-    s_vtx[wv][lane] = v;
-    // window positions are unsigned: the lists of one wavefront may hold
-    // nearly 2^31 entries, and a step past the last must not wrap
-    const unsigned utotal = (unsigned)total;
-    for (unsigned w0 = 0; w0 < utotal; w0 += kWaveWindow) {
-      const unsigned wend = w0 + kWaveWindow < utotal ? w0 + kWaveWindow : utotal;
-      for (unsigned long long m = bits; m; m &= m - 1) {
-        const int L = __ffsll((long long)m) - 1;
-        const int oL = __builtin_amdgcn_readlane(off, L);
-        const int dL = __builtin_amdgcn_readlane(deg, L);
-        const unsigned a = (unsigned)oL > w0 ? (unsigned)oL : w0;
-        const unsigned b = (unsigned)(oL + dL) < wend ? (unsigned)(oL + dL) : wend;
-        for (unsigned x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const GraphRef G = resolve<BATCH>(p, ws0);
+    int* const ns = G.nstat;
+    const int* const nl = G.nlist;
+    unsigned char* const owner = s_owner[wv];
+    int2* const surv = s_surv[wv];
+    const int R = p.filter ? p.giant[G.g] : -1;
+    const bool skip = p.filter == 2 && R >= 0;  // listed vertices lie outside R's tree
+    int nsurv = 0;  // wave-uniform
+    unsigned long long intree = 0;  // per lane: owners seen with an entry inside R's tree
+    // groups of up to 64 listed vertices, one per lane
+    for (int g0 = 0; g0 < cnt; g0 += 64) {
+      const int gn = cnt - g0 < 64 ? cnt - g0 : 64;
+      const unsigned long long bits = gn == 64 ? ~0ull : ((1ull << gn) - 1ull);
+      const bool mine = lane < gn;
+      const long long slot = ws0 + (mine ? list[g0 + lane] : 0);
+      const int v = (int)(slot - G.slot0);
+      // a marked vertex is joined to the sampled root: that root is a valid
+      // estimate (the CAS of a hook checks it), no walk needed
+      const bool marked = mine && p.filter && !skip && slot_bit(p.gbits, slot);
+      const unsigned long long mk = __ballot(marked);
+      int beg = 0, deg = 0, root = 0;
+      if (mine) {
+        root = marked ? R : lab_ld(&ns[v]);
+        beg = G.nidx[v];
+        deg = G.nidx[v + 1] - beg;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (unsigned e0 = w0; e0 < wend; e0 += 256) {
-        int u[4], src[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const unsigned e = e0 + k * 64 + lane;
-          const bool valid = e < wend;
-          src[k] = valid ? owner[e - w0] : 0;
-          u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
+      if (mine && !marked) root = find_from(v, root, ns, p.path_mode);
+      const int incl = (int)wave_incl_sum((unsigned)deg);
+      const int off = incl - deg;
+      const int total = __shfl(incl, 63);
+      s_base[wv][lane] = beg - off;  // entry e of lane L's list is nlist[base_L + e]
+      s_root[wv][lane] = root;
+      s_vtx[wv][lane] = v;
+      // window positions are unsigned: the lists of one wavefront may hold
+      // nearly 2^31 entries, and a step past the last must not wrap
+      const unsigned utotal = (unsigned)total;
+      for (unsigned w0 = 0; w0 < utotal; w0 += kWaveWindow) {
+        const unsigned wend = w0 + kWaveWindow < utotal ? w0 + kWaveWindow : utotal;
+        for (unsigned long long m = bits; m; m &= m - 1) {
+          const int L = __ffsll((long long)m) - 1;
+          const int oL = __builtin_amdgcn_readlane(off, L);
+          const int dL = __builtin_amdgcn_readlane(deg, L);
+          const unsigned a = (unsigned)oL > w0 ? (unsigned)oL : w0;
+          const unsigned b = (unsigned)(oL + dL) < wend ? (unsigned)(oL + dL) : wend;
+          for (unsigned x = a + lane; x < b; x += 64) owner[x - w0] = (unsigned char)L;
         }
-        unsigned keep = 0;
-        if (skip) {
-          // every entry; a neighbour inside the largest tree stands for R,
-          // and all such entries of one vertex join the same two trees: the
-          // owner is noted and hooked to R once, after its list
-#pragma unroll
-          for (int k = 0; k < 4; k++)
-            keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (unsigned e0 = w0; e0 < wend; e0 += 256) {
+          int u[4], src[4];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
-            if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) {
-              keep &= ~(1u << k);
-              intree |= 1ull << src[k];
-            }
+            const unsigned e = e0 + k * 64 + lane;
+            const bool valid = e < wend;
+            src[k] = valid ? owner[e - w0] : 0;
+            u[k] = valid ? ld_stream(&nl[s_base[wv][src[k]] + e]) : INT_MAX;
           }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
-          if (mk) {
-            unsigned in = 0;
+          unsigned keep = 0;
+          if (skip) {
+            // every entry; a neighbour inside the largest tree stands for R,
+            // and all such entries of one vertex join the same two trees: the
+            // owner is noted and hooked to R once, after its list
 #pragma unroll
             for (int k = 0; k < 4; k++)
-              if (((keep >> k) & 1u) && ((mk >> src[k]) & 1ull))
-                in |= slot_bit(p.gbits, G.slot0 + u[k]) << k;
-            keep &= ~in;
-          }
-        }
-        // stage the entries that still need a hook; flush when the stage is full
-        const unsigned c = __popc(keep);
-        const unsigned pos = wave_incl_sum(c);
-        const int wtot = (int)__shfl(pos, 63);
-        if (wtot) {
-          if (nsurv + wtot > kWaveSurvivors) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            flush_wave(surv, nsurv, ns, p.path_mode);
-            nsurv = 0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          }
-          int at = nsurv + (int)(pos - c);
+              keep |= (u[k] != INT_MAX && u[k] != s_vtx[wv][src[k]]) ? (1u << k) : 0u;
 #pragma unroll
-          for (int k = 0; k < 4; k++)
-            if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
-          nsurv += wtot;
+            for (int k = 0; k < 4; k++) {
+              if (((keep >> k) & 1u) && slot_bit(p.gbits, G.slot0 + u[k])) {
+                keep &= ~(1u << k);
+                intree |= 1ull << src[k];
+              }
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) keep |= (s_vtx[wv][src[k]] > u[k]) ? (1u << k) : 0u;
+            if (mk) {
+              unsigned in = 0;
+#pragma unroll
+              for (int k = 0; k < 4; k++)
+                if (((keep >> k) & 1u) && ((mk >> src[k]) & 1ull))
+                  in |= slot_bit(p.gbits, G.slot0 + u[k]) << k;
+              keep &= ~in;
+            }
+          }
+          // stage the entries that still need a hook; flush when the stage is full
+          const unsigned c = __popc(keep);
+          const unsigned pos = wave_incl_sum(c);
+          const int wtot = (int)__shfl(pos, 63);
+          if (wtot) {
+            if (nsurv + wtot > kWaveSurvivors) {
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+              flush_wave(surv, nsurv, ns, p.path_mode);
+              nsurv = 0;
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            int at = nsurv + (int)(pos - c);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if ((keep >> k) & 1u) surv[at++] = make_int2(u[k], s_root[wv][src[k]]);
+            nsurv += wtot;
+          }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    if (skip) {
-      // one hook to R per listed vertex with a neighbour in R's tree
+      if (skip) {
+        // one hook to R per listed vertex with a neighbour in R's tree
 #pragma unroll
-      for (int d = 32; d > 0; d >>= 1) intree |= __shfl_xor(intree, d);
-      if ((intree >> lane) & 1ull) (void)hook_edge(root, R, lab_ld(&ns[R]), ns, p.path_mode);
-      intree = 0;
+        for (int d = 32; d > 0; d >>= 1) intree |= __shfl_xor(intree, d);
+        if ((intree >> lane) & 1ull) (void)hook_edge(root, R, lab_ld(&ns[R]), ns, p.path_mode);
+        intree = 0;
+      }
     }
+    if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
+    // the next chunk rewrites the LDS tables
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (nsurv) flush_wave(surv, nsurv, ns, p.path_mode);
 }
 
 // ---- compute3 (ECL-CC-atomic.cu:203-246) --------------------------------------
@@ -1475,8 +1489,8 @@ static unsigned spread_stride(unsigned blocks) {
 // Blocks a vertex-parallel launch should have at least (8 per CU), where
 // its per-block share of words can shrink.
 constexpr long long kMinBlocks = 2048;
-// Wavefronts a two-chunk launch must have at least: two rounds of the GPU's
-// 8192 resident wavefronts (256 CUs x 4 SIMDs x 8).
+// Wavefronts a launch with several chunks per wavefront must have at least:
+// two rounds of the GPU's 8192 resident wavefronts (256 CUs x 4 SIMDs x 8).
 constexpr long long kTwoChunkWaves = 16384;
 
 template <bool BATCH>
@@ -1528,7 +1542,19 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   while (span < kLowWords && words / (2 * span) >= kMinBlocks) span *= 2;
   k_hook_thread<BATCH><<<dim3((unsigned)((words + span - 1) / span)), block, 0, s>>>(p, span);
   mark_ev();
-  k_hook_wave<BATCH><<<dim3((unsigned)((p.nslots + 4 * kSlotAlign - 1) / (4 * kSlotAlign))), block, 0, s>>>(p);
+  {
+    // chunks per compute2 wavefront: the most of 16, 4, 1 that still leaves
+    // two rounds of resident wavefronts.  Most chunks of a batch list no
+    // medium vertex (4096-graph mix: 0.834 ms at one chunk, 0.602 at four,
+    // 0.500 at sixteen); R-MAT-24's are busier and want the wavefronts
+    // (0.0226 ms at one, 0.0186 at four, 0.0366 at sixteen)
+    const long long chunks = (p.nslots + kSlotAlign - 1) / kSlotAlign;
+    const int gc = chunks / 16 >= kTwoChunkWaves ? 16 : chunks / 4 >= kTwoChunkWaves ? 4 : 1;
+    const dim3 cgrid((unsigned)((chunks + 4 * gc - 1) / (4 * gc)));
+    if (gc == 16) k_hook_wave<BATCH, 16><<<cgrid, block, 0, s>>>(p);
+    else if (gc == 4) k_hook_wave<BATCH, 4><<<cgrid, block, 0, s>>>(p);
+    else k_hook_wave<BATCH, 1><<<cgrid, block, 0, s>>>(p);
+  }
   mark_ev();
   {
     // filter 2 too: the lists of the high-degree vertices mark left listed
