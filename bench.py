@@ -579,6 +579,9 @@ def main() -> None:
     # batched / served
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--tune-graphs", type=int, default=256)
+    ap.add_argument("--no-per-graph", action="store_true",
+                    help="batched mode: leave out the one-graph-per-launch runs (for counter runs, "
+                         "where the profiler's cost per dispatch would dominate)")
     ap.add_argument("--tune-budget", type=float, default=0.0,
                     help="seconds of autotuning on a sample of the mix; 0 = the committed table")
     ap.add_argument("--requests", type=int, default=1024)

@@ -144,4 +144,61 @@ int check_hip(hipError_t e, int code, const char* what);
 int resolve_tuning(const eclcc_params* in, Tuning* out);
 int device_cu_count();
 
+// Block order of the batched walk kernels (a permutation of gridDim.x work
+// items; the kernels take item order_item(o, blockIdx.x, gridDim.x)).  The
+// items may be cut into windows of 2^wlog consecutive items, each permuted
+// on its own and taken one after the other; the last, partial window (or
+// the whole grid, with wlog 32) is permuted by a stride coprime with its
+// size.
+struct BlockOrder {
+  unsigned wlog;     // log2 of the items per window; 32: one window
+  unsigned wstride;  // odd
+  unsigned tstride;  // coprime with the last window's size
+};
+
+__host__ __device__ inline unsigned order_item(const BlockOrder o, unsigned b, unsigned grid) {
+  const unsigned full = o.wlog < 32u ? (grid >> o.wlog) << o.wlog : 0u;
+  if (b < full) {
+    const unsigned m = (1u << o.wlog) - 1u;
+    // b * wstride wraps mod 2^32, a multiple of 2^wlog: the window's base
+    // contributes nothing to the offset
+    return (b & ~m) | ((b * o.wstride) & m);
+  }
+  const unsigned r = grid - full;
+  return full + (unsigned)(((unsigned long long)(b - full) * o.tstride) % r);
+}
+
+// A stride coprime with `blocks` near its golden-ratio fraction.
+inline unsigned spread_stride(unsigned blocks) {
+  if (blocks <= 2) return 1;
+  unsigned x = (unsigned)((double)blocks * 0.6180339887) | 1u;
+  for (;; x += 2) {
+    unsigned a = x, b = blocks;
+    while (b) {
+      const unsigned t = a % b;
+      a = b;
+      b = t;
+    }
+    if (a == 1) return x;
+  }
+}
+
+// The order of `blocks` work items of `slots` slots each, in windows whose
+// labels take at most `window_bytes` (0: one window over the whole grid).
+inline BlockOrder block_order(unsigned blocks, long long slots, long long window_bytes) {
+  BlockOrder o{32u, 1u, 1u};
+  const long long items = window_bytes / (4 * slots);
+  unsigned lg = 0;
+  while (lg < 30 && (2ll << lg) <= items) lg++;
+  if (items >= 2 && blocks > (1u << lg)) {
+    o.wlog = lg;
+    o.wstride = (unsigned)((double)(1u << lg) * 0.6180339887) | 1u;
+    const unsigned full = (blocks >> lg) << lg;
+    o.tstride = blocks > full ? spread_stride(blocks - full) : 1u;
+  } else {
+    o.tstride = spread_stride(blocks);
+  }
+  return o;
+}
+
 }  // namespace eclcc

@@ -164,15 +164,20 @@ __device__ __forceinline__ GraphRef resolve(const KParams& p, long long slot) {
   else return graph_ref<true>(p, p.chunk_graph[slot >> 8]);
 }
 
-// Block order of the batched walk kernels: block b takes work item
-// (b * stride) mod gridDim.x, stride coprime with the grid (a permutation),
-// so that the blocks resident at one time are spread over the whole batch
-// rather than crowded into a few consecutive graphs, whose top-of-tree
-// labels and roots every walk of the graph reaches (1024-graph batch of the
-// ER / R-MAT mix: sample 5.79 -> 5.49 ms; grouping each graph's blocks on
-// one XCD instead: 6.02 ms).
-__device__ __forceinline__ unsigned spread_block(unsigned stride) {
-  return (unsigned)(((unsigned long long)blockIdx.x * stride) % gridDim.x);
+// Block order of the batched walk kernels: a permutation of the work items
+// that spreads the blocks resident at one time over many graphs rather
+// than a few consecutive ones, whose top-of-tree labels and roots every
+// walk of the graph reaches (1024-graph batch of the ER / R-MAT mix: sample
+// 5.79 -> 5.49 ms; grouping each graph's blocks on one XCD instead: 6.02
+// ms).  The items may be cut into windows of 2^wlog consecutive items,
+// each permuted on its own (item = window base + (b * wstride) mod 2^wlog,
+// wstride odd) and taken one after the other; the last, partial window
+// (or the whole grid, with wlog 32) takes (b * tstride) mod its size,
+// tstride coprime with that size (BlockOrder, order_item).  A window whose
+// labels fit the 256 MB Infinity Cache keeps mark's walks there (see
+// launch_all).
+__device__ __forceinline__ unsigned spread_block(const BlockOrder o) {
+  return order_item(o, blockIdx.x, gridDim.x);
 }
 
 // first slot of this thread's wavefront, made wave-uniform
@@ -556,7 +561,7 @@ __global__ __launch_bounds__(256) void k_sample_lane(KParams p) {
 constexpr int kSampleWords = 128;
 
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride, int span) {
+__global__ __launch_bounds__(256) void k_sample_dense(KParams p, BlockOrder order, int span) {
   static_assert(kSampleWords <= 128, "one word per lane of the first two wavefronts");
   constexpr int kChunks = kSampleWords * 64 / kSlotAlign;
   __shared__ unsigned short s_list[kSampleWords * 64];
@@ -566,7 +571,7 @@ __global__ __launch_bounds__(256) void k_sample_dense(KParams p, unsigned stride
   __shared__ int* s_ns[kChunks];
   __shared__ long long s_s0[kChunks];
   const long long words = (p.nslots + 63) / 64;
-  const long long w0 = (long long)spread_block(stride) * span;
+  const long long w0 = (long long)spread_block(order) * span;
   const int tid = (int)threadIdx.x;
   if (tid < 128) {
     // the block's words, one per thread of the first two wavefronts, and
@@ -966,11 +971,11 @@ __global__ __launch_bounds__(256) void k_mark_lane(KParams p) {
 // 1024-graph batch: 1.71 -> 1.52 ms at four per lane; 4096-graph mix: 6.14
 // -> 5.75 ms at eight).  The launch takes CPW = 2 for large batches only.
 template <bool BATCH, int CPW>
-__global__ __launch_bounds__(256) void k_mark(KParams p, unsigned stride) {
+__global__ __launch_bounds__(256) void k_mark(KParams p, BlockOrder order) {
   constexpr int H = kSlotAlign / 64;  // words per chunk
   constexpr int K = CPW * H;          // CPW chunks per wavefront
   const long long ws0 =
-      ((long long)spread_block(stride) * 4 + uniform((int)(threadIdx.x >> 6))) * (CPW * kSlotAlign);
+      ((long long)spread_block(order) * 4 + uniform((int)(threadIdx.x >> 6))) * (CPW * kSlotAlign);
   if (ws0 >= p.nslots) return;
   const long long nwords = (p.nslots + 63) / 64;
   const GraphRef G0 = resolve<BATCH>(p, ws0);
@@ -1471,20 +1476,15 @@ __global__ __launch_bounds__(256) void k_flatten(KParams p) {
 
 // ---- host-side launch -----------------------------------------------------------
 
-// A stride coprime with `blocks` near its golden-ratio fraction.
-static unsigned spread_stride(unsigned blocks) {
-  if (blocks <= 2) return 1;
-  unsigned x = (unsigned)((double)blocks * 0.6180339887) | 1u;
-  for (;; x += 2) {
-    unsigned a = x, b = blocks;
-    while (b) {
-      const unsigned t = a % b;
-      a = b;
-      b = t;
-    }
-    if (a == 1) return x;
-  }
-}
+// Labels per window of the batched mark: its walks mostly read, and a
+// window of 32 MB of labels keeps them in the Infinity Cache while still
+// spreading the resident blocks over hundreds of graphs (4096-graph mix,
+// mark ms at windows of 4 / 8 / 16 / 32 / 128 / 512 MB and one window:
+// 4.92 / 4.72 / 4.61 / 4.51 / 4.86 / 5.11 / 5.76).  Sample keeps one
+// window: its hooks end in a CAS on each graph's few roots, and windows
+// crowd them (7.39 ms at one window, 7.48 at 512 MB, 8.05 at 128, 9.27 at
+// 32).
+constexpr long long kMarkWindowBytes = 32ll << 20;
 
 // Blocks a vertex-parallel launch should have at least (8 per CU), where
 // its per-block share of words can shrink.
@@ -1520,7 +1520,7 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
     int sspan = kSampleWords;
     while (sspan > 1 && (words + sspan - 1) / sspan < kMinBlocks) sspan /= 2;
     const unsigned nb = (unsigned)((words + sspan - 1) / sspan);
-    k_sample_dense<true><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb), sspan);
+    k_sample_dense<true><<<dim3(nb), block, 0, s>>>(p, block_order(nb, 64ll * sspan, 0), sspan);
   } else {
     k_sample_lane<<<grid, block, 0, s>>>(p);
   }
@@ -1529,8 +1529,9 @@ static hipError_t launch_all(const KParams& p, const Shape& shape, hipStream_t s
   mark_ev();
   if constexpr (BATCH) {
     const unsigned nb = (unsigned)((p.nslots + 4 * cpw * kSlotAlign - 1) / (4 * cpw * kSlotAlign));
-    if (cpw == 2) k_mark<true, 2><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
-    else k_mark<true, 1><<<dim3(nb), block, 0, s>>>(p, spread_stride(nb));
+    const BlockOrder order = block_order(nb, 4ll * cpw * kSlotAlign, kMarkWindowBytes);
+    if (cpw == 2) k_mark<true, 2><<<dim3(nb), block, 0, s>>>(p, order);
+    else k_mark<true, 1><<<dim3(nb), block, 0, s>>>(p, order);
   } else {
     k_mark_lane<<<grid, block, 0, s>>>(p);
   }

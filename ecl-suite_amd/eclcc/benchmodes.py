@@ -113,23 +113,25 @@ def batched(args, check=None) -> dict:
     # with the built-in defaults: the fastest parameters for graphs of this
     # mix run one at a time (13.4 K graphs/s on the served mix, against 9.1 K
     # with the batch's tuned path_mode 2, which pays off only in full batches)
-    log("one graph per launch")
-    b.set_params(BASELINE_PARAMS())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for g in range(count):
-        b.run_single(g, stream)
+    plain_sync_s = plain_async_s = same = None
+    if not getattr(args, "no_per_graph", False):
+        log("one graph per launch")
+        b.set_params(BASELINE_PARAMS())
         torch.cuda.synchronize()
-    plain_sync_s = time.perf_counter() - t0
-    lab_plain = b.all_labels()
-    # one graph per launch, all enqueued back to back (no host wait)
-    start.record(stream)
-    for g in range(count):
-        b.run_single(g, stream)
-    end.record(stream)
-    torch.cuda.synchronize()
-    plain_async_s = start.elapsed_time(end) * 1e-3
-    same = _labels_equal(lab_batch, lab_plain) and _labels_equal(lab_plain, b.all_labels())
+        t0 = time.perf_counter()
+        for g in range(count):
+            b.run_single(g, stream)
+            torch.cuda.synchronize()
+        plain_sync_s = time.perf_counter() - t0
+        lab_plain = b.all_labels()
+        # one graph per launch, all enqueued back to back (no host wait)
+        start.record(stream)
+        for g in range(count):
+            b.run_single(g, stream)
+        end.record(stream)
+        torch.cuda.synchronize()
+        plain_async_s = start.elapsed_time(end) * 1e-3
+        same = _labels_equal(lab_batch, lab_plain) and _labels_equal(lab_plain, b.all_labels())
     B = algorithmic_bytes(graphs)
     gps = count / (ms * 1e-3)
     out = {
@@ -152,11 +154,11 @@ def batched(args, check=None) -> dict:
                    "params": best, "hipgraph": True},
         "edges_per_s": m_tot / (ms * 1e-3),
         "pct_hbm_peak": 100.0 * B / (ms * 1e-3) / HBM_PEAK,
-        "plain_sync_graphs_per_s": count / plain_sync_s,
-        "plain_async_graphs_per_s": count / plain_async_s,
-        "speedup_vs_plain_sync": (count / (ms * 1e-3)) / (count / plain_sync_s),
-        "speedup_vs_plain_async": (count / (ms * 1e-3)) / (count / plain_async_s),
-        "labels_batched_equal_plain": bool(same),
+        "plain_sync_graphs_per_s": count / plain_sync_s if plain_sync_s else None,
+        "plain_async_graphs_per_s": count / plain_async_s if plain_async_s else None,
+        "speedup_vs_plain_sync": gps / (count / plain_sync_s) if plain_sync_s else None,
+        "speedup_vs_plain_async": gps / (count / plain_async_s) if plain_async_s else None,
+        "labels_batched_equal_plain": None if same is None else bool(same),
         "labels_equal_oracle": oracle_ok,
         "params_source": params_source,
         "baseline_params": BASELINE_PARAMS().as_dict(),
