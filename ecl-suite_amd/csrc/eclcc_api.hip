@@ -457,19 +457,18 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
     hipStream_t s;
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } sg{s};
-  // host CSR in: large arrays through pinned staging (upload_host)
-  if ((rc = upload_host(d_idx, nindex, sizeof(int32_t) * ((size_t)nodes + 1), s))) return rc;
-  if ((rc = upload_host(d_lst, nlist, sizeof(int32_t) * (size_t)edges, s))) return rc;
+  // host CSR in, both arrays in one pass through pinned staging
+  const CopySpan csr[2] = {{d_idx, const_cast<int32_t*>(nindex), sizeof(int32_t) * ((size_t)nodes + 1)},
+                           {d_lst, const_cast<int32_t*>(nlist), sizeof(int32_t) * (size_t)edges}};
+  if ((rc = upload_host(csr, 2, s))) return rc;
   float kms[kNumKernels], tot = 0.f;
   rc = eclcc_run_device_profiled(nodes, edges, (const int32_t*)d_idx, (const int32_t*)d_lst,
                                  (int32_t*)d_st, d_w, wbytes, params, s, kms, &tot);
   if (rc) return rc;
   if (seconds) *seconds = 0.001 * (double)tot;
-  if ((rc = check_hip(hipMemcpyAsync(nstat, d_st, sizeof(int32_t) * (size_t)nodes,
-                                     hipMemcpyDeviceToHost, s),
-                      ECLCC_ERR_COPY, "copying from device failed")))
-    return rc;
-  return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+  // labels out through pinned staging too: a pageable copy back to the host
+  // ran at about 7 GB/s (64 MiB in 9.5 ms)
+  return download_host(nstat, d_st, sizeof(int32_t) * (size_t)nodes, s);
 } catch (...) {
   return host_exception();
 }

@@ -1,19 +1,21 @@
-// eclcc_upload.hip -- host-to-device copies of large pageable arrays for the
-// drop-in call (eclcc_compute), which takes the caller's host CSR as the
-// reference's computeCC does (ECL-CC-atomic.cu:295-296).
+// eclcc_upload.hip -- copies between large pageable host arrays and the
+// device for the drop-in call (eclcc_compute), which takes the caller's host
+// CSR and fills the caller's host labels as the reference's computeCC does
+// (ECL-CC-atomic.cu:295-296, 318).
 //
-// A copy from pageable memory is staged by the runtime and ran at 27-30
-// GB/s on the MI355X box (2.08 GB of R-MAT-24 entries in 70-76 ms), while a
-// copy from pinned memory runs at 57.6 GB/s.  Pinning the caller's array in
+// On the MI355X box a copy from pageable memory is staged by the runtime and
+// ran at 27-30 GB/s (2.08 GB of R-MAT-24 entries in 70-76 ms), a copy from
+// pinned memory at 57.6 GB/s, and a copy of 64 MiB of labels back into
+// pageable memory at about 7 GB/s (9.5 ms).  Pinning the caller's arrays in
 // place (hipHostRegister) is not available here, so large copies go through
 // a pool of pinned staging slots instead: kStageThreads host threads each
-// copy every kStageThreads-th chunk of the source into one of their own
-// slots and send it to the device on their own stream, so the host copies
-// of some chunks overlap the DMA of others.  The pool (kStageThreads x
-// kStageSlots x kStageChunk = 128 MiB of pinned memory, and one stream per
-// thread) is made on the first large copy and held until the process ends;
-// one large copy runs at a time.  Small copies, and any copy when the pool
-// cannot be made, use the plain pageable path.
+// take every kStageThreads-th chunk and move it through one of their own
+// slots on their own stream, so the host copies of some chunks overlap the
+// DMA of others.  The pool (kStageThreads x kStageSlots x kStageChunk = 128
+// MiB of pinned memory, and two streams per thread) is made on the first
+// large copy and held until the process ends; one staged copy runs at a
+// time.  Small copies, and any copy when the pool cannot be made, use the
+// plain pageable path.
 
 #include <hip/hip_runtime.h>
 
@@ -32,8 +34,11 @@ namespace {
 constexpr size_t kStageChunk = size_t(8) << 20;
 constexpr int kStageThreads = 8;
 constexpr int kStageSlots = 2;
-// below this, one pageable copy costs less than waking the pool
+// below these, one pageable copy costs less than waking the pool (a
+// pageable copy back to the host is four times slower than one to the
+// device, so downloads are staged from a smaller size)
 constexpr size_t kStageMinBytes = size_t(64) << 20;
+constexpr size_t kStageMinDownBytes = size_t(8) << 20;
 
 struct StagePool {
   std::mutex mu;  // one staged copy at a time
@@ -41,8 +46,10 @@ struct StagePool {
   bool ok = false;
   int device = -1;
   char* buf = nullptr;
-  hipStream_t streams[kStageThreads] = {};
-  hipEvent_t done[kStageThreads][kStageSlots] = {};
+  // uploads and downloads use streams and events of their own (the slots
+  // are shared: every staged copy is complete before the next begins)
+  hipStream_t streams[2][kStageThreads] = {};
+  hipEvent_t done[2][kStageThreads][kStageSlots] = {};
 };
 
 StagePool& pool() {
@@ -61,61 +68,42 @@ bool make_pool(StagePool& p) {
     return false;
   }
   p.buf = static_cast<char*>(b);
-  for (int t = 0; t < kStageThreads; t++) {
-    if (hipStreamCreateWithFlags(&p.streams[t], hipStreamNonBlocking) != hipSuccess) return false;
-    for (int k = 0; k < kStageSlots; k++)
-      if (hipEventCreateWithFlags(&p.done[t][k], hipEventDisableTiming) != hipSuccess) return false;
-  }
+  for (int d = 0; d < 2; d++)
+    for (int t = 0; t < kStageThreads; t++) {
+      if (hipStreamCreateWithFlags(&p.streams[d][t], hipStreamNonBlocking) != hipSuccess) return false;
+      for (int k = 0; k < kStageSlots; k++)
+        if (hipEventCreateWithFlags(&p.done[d][t][k], hipEventDisableTiming) != hipSuccess) return false;
+    }
   p.ok = true;
   return true;
 }
 
-}  // namespace
+// One chunk of a staged copy: `len` bytes between host and device.
+struct Chunk {
+  char* dev;
+  char* host;
+  size_t len;
+};
 
-int upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if (bytes == 0) return ECLCC_OK;
-  StagePool& p = pool();
-  std::unique_lock<std::mutex> lock(p.mu, std::defer_lock);
-  int dev = -1;
-  bool staged = bytes >= kStageMinBytes && hipGetDevice(&dev) == hipSuccess;
-  if (staged) {
-    lock.lock();
-    staged = make_pool(p) && p.device == dev;
-    if (!staged) lock.unlock();
-  }
-  if (!staged)
-    return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), ECLCC_ERR_COPY,
-                     "copying to device failed");
-  // work queued on `s` before the call finishes before the staged copies
-  int rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying to device failed");
-  if (rc) return rc;
-  const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
-  std::atomic<int> err{hipSuccess};
-  auto worker = [&](int t) {
-    (void)hipSetDevice(p.device);
-    size_t round = 0;
-    for (size_t c = (size_t)t; c < nchunks && err.load() == hipSuccess; c += kStageThreads, round++) {
-      const int k = (int)(round % kStageSlots);
-      char* slot = p.buf + ((size_t)t * kStageSlots + k) * kStageChunk;
-      hipError_t e = hipEventSynchronize(p.done[t][k]);  // the slot's last DMA
-      const size_t off = c * kStageChunk;
-      const size_t len = bytes - off < kStageChunk ? bytes - off : kStageChunk;
-      if (e == hipSuccess) {
-        std::memcpy(slot, static_cast<const char*>(src) + off, len);
-        e = hipMemcpyAsync(static_cast<char*>(dst) + off, slot, len, hipMemcpyHostToDevice, p.streams[t]);
-      }
-      if (e == hipSuccess) e = hipEventRecord(p.done[t][k], p.streams[t]);
-      if (e != hipSuccess) {
-        int expect = hipSuccess;
-        err.compare_exchange_strong(expect, (int)e);
-      }
-    }
-    const hipError_t e = hipStreamSynchronize(p.streams[t]);
-    if (e != hipSuccess) {
-      int expect = hipSuccess;
-      err.compare_exchange_strong(expect, (int)e);
-    }
-  };
+std::vector<Chunk> cut(const CopySpan* spans, int nspans) {
+  std::vector<Chunk> out;
+  for (int i = 0; i < nspans; i++)
+    for (size_t off = 0; off < spans[i].bytes; off += kStageChunk)
+      out.push_back(Chunk{static_cast<char*>(spans[i].dev) + off, static_cast<char*>(spans[i].host) + off,
+                          spans[i].bytes - off < kStageChunk ? spans[i].bytes - off : kStageChunk});
+  return out;
+}
+
+void note(std::atomic<int>& err, hipError_t e) {
+  if (e == hipSuccess) return;
+  int expect = hipSuccess;
+  err.compare_exchange_strong(expect, (int)e);
+}
+
+// Runs worker(t) on kStageThreads threads (this one included); the first
+// error any of them noted, as a status.
+template <class F>
+int run_workers(std::atomic<int>& err, F worker) {
   std::vector<std::thread> th;
   th.reserve(kStageThreads - 1);
   try {
@@ -126,8 +114,107 @@ int upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
   worker(0);
   for (std::thread& x : th) x.join();
   if (err.load() != hipSuccess)
-    return check_hip((hipError_t)err.load(), ECLCC_ERR_COPY, "copying to device failed");
+    return check_hip((hipError_t)err.load(), ECLCC_ERR_COPY, "copying between host and device failed");
   return ECLCC_OK;
+}
+
+// Locks and makes the pool for a copy of `bytes` (total) at least `min`
+// bytes long; false (unlocked) when the plain path is to be taken.
+bool take_pool(StagePool& p, std::unique_lock<std::mutex>& lock, size_t bytes, size_t min) {
+  int dev = -1;
+  if (bytes < min || hipGetDevice(&dev) != hipSuccess) return false;
+  lock.lock();
+  if (make_pool(p) && p.device == dev) return true;
+  lock.unlock();
+  return false;
+}
+
+}  // namespace
+
+int upload_host(const CopySpan* spans, int nspans, hipStream_t s) {
+  size_t total = 0;
+  for (int i = 0; i < nspans; i++) total += spans[i].bytes;
+  if (total == 0) return ECLCC_OK;
+  StagePool& p = pool();
+  std::unique_lock<std::mutex> lock(p.mu, std::defer_lock);
+  if (!take_pool(p, lock, total, kStageMinBytes)) {
+    for (int i = 0; i < nspans; i++)
+      if (spans[i].bytes) {
+        const int rc = check_hip(hipMemcpyAsync(spans[i].dev, spans[i].host, spans[i].bytes,
+                                                hipMemcpyHostToDevice, s),
+                                 ECLCC_ERR_COPY, "copying to device failed");
+        if (rc) return rc;
+      }
+    return ECLCC_OK;
+  }
+  // work queued on `s` before the call finishes before the staged copies
+  int rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying to device failed");
+  if (rc) return rc;
+  const std::vector<Chunk> chunks = cut(spans, nspans);
+  std::atomic<int> err{hipSuccess};
+  return run_workers(err, [&](int t) {
+    (void)hipSetDevice(p.device);
+    size_t round = 0;
+    for (size_t c = (size_t)t; c < chunks.size() && err.load() == hipSuccess; c += kStageThreads, round++) {
+      const int k = (int)(round % kStageSlots);
+      char* slot = p.buf + ((size_t)t * kStageSlots + k) * kStageChunk;
+      hipError_t e = hipEventSynchronize(p.done[0][t][k]);  // the slot's last DMA
+      if (e == hipSuccess) {
+        std::memcpy(slot, chunks[c].host, chunks[c].len);
+        e = hipMemcpyAsync(chunks[c].dev, slot, chunks[c].len, hipMemcpyHostToDevice, p.streams[0][t]);
+      }
+      if (e == hipSuccess) e = hipEventRecord(p.done[0][t][k], p.streams[0][t]);
+      note(err, e);
+    }
+    note(err, hipStreamSynchronize(p.streams[0][t]));
+  });
+}
+
+int upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  const CopySpan one{dst, const_cast<void*>(src), bytes};
+  return upload_host(&one, 1, s);
+}
+
+int download_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+  StagePool& p = pool();
+  std::unique_lock<std::mutex> lock(p.mu, std::defer_lock);
+  if (!take_pool(p, lock, bytes, kStageMinDownBytes)) {
+    const int rc = check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), ECLCC_ERR_COPY,
+                             "copying from device failed");
+    if (rc) return rc;
+    return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+  }
+  // the data is written by work on `s`
+  int rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+  if (rc) return rc;
+  const CopySpan one{const_cast<void*>(src), dst, bytes};
+  const std::vector<Chunk> chunks = cut(&one, 1);
+  std::atomic<int> err{hipSuccess};
+  return run_workers(err, [&](int t) {
+    (void)hipSetDevice(p.device);
+    // this thread's chunks are t, t + T, t + 2T, ...; chunk i of them moves
+    // through slot i mod kStageSlots, its DMA issued kStageSlots chunks ahead
+    auto issue = [&](size_t i) {
+      const size_t c = (size_t)t + i * kStageThreads;
+      char* slot = p.buf + ((size_t)t * kStageSlots + i % kStageSlots) * kStageChunk;
+      hipError_t e = hipMemcpyAsync(slot, chunks[c].dev, chunks[c].len, hipMemcpyDeviceToHost, p.streams[1][t]);
+      if (e == hipSuccess) e = hipEventRecord(p.done[1][t][i % kStageSlots], p.streams[1][t]);
+      note(err, e);
+    };
+    const size_t mine = chunks.size() > (size_t)t ? (chunks.size() - (size_t)t + kStageThreads - 1) / kStageThreads : 0;
+    for (size_t i = 0; i < mine && i < (size_t)kStageSlots; i++) issue(i);
+    for (size_t i = 0; i < mine && err.load() == hipSuccess; i++) {
+      const size_t c = (size_t)t + i * kStageThreads;
+      const hipError_t e = hipEventSynchronize(p.done[1][t][i % kStageSlots]);
+      note(err, e);
+      if (e != hipSuccess) break;
+      std::memcpy(chunks[c].host, p.buf + ((size_t)t * kStageSlots + i % kStageSlots) * kStageChunk,
+                  chunks[c].len);
+      if (i + kStageSlots < mine) issue(i + kStageSlots);
+    }
+    note(err, hipStreamSynchronize(p.streams[1][t]));
+  });
 }
 
 }  // namespace eclcc
