@@ -444,6 +444,7 @@ def mst_bench(args) -> dict:
     spanning = bool(int(sel.sum()) == g.nodes - ncomp and np.array_equal(df.labels(), lab))
     del df
     weight = int(w[sel].astype(np.int64).sum())
+    host_call = mst_host_array_call(g, w, forest)
     # edge-for-edge parity and the CPU baseline on a bounded sample
     gs = workloads.rmat(21)
     ws = mst.mst_weights(gs)
@@ -492,6 +493,7 @@ def mst_bench(args) -> dict:
         "components": ncomp,
         "spanning_forest": spanning,
         "sample_parity": {"workload": gs.name, "edges_equal_oracle": bool(np.array_equal(got_s, ref))},
+        "host_array_call": host_call,
         "cpu_baseline": {
             "value": gs.edges / statistics.median(times),
             "unit": "edges/s",
@@ -512,6 +514,47 @@ def mst_bench(args) -> dict:
             out["pass_roofline"]["traffic_frac"] = gbs * 1e9 / HBM_PEAK
             out["traffic_source"] = f"profiles/{tr.name} (rocprofv3: {t.get('source', 'FETCH_SIZE + WRITE_SIZE, copy-calibrated')})"
     return out
+
+
+def mst_host_array_call(g, w, forest, calls: int = 3) -> dict:
+    """The drop-in eclmst_compute on host arrays, as the reference's main
+    calls gpuMST (ECL-MST_10-atomic.cu:450-462): allocate, copy nindex,
+    nlist and weights in, run, copy in_mst out, free.  Wall time per call
+    (median of `calls` after one warm-up call), against the PCIe link; the
+    forest must equal the device-resident run's."""
+    import ctypes as C
+
+    import numpy as np
+    from eclcc import _native as N
+
+    ni = np.ascontiguousarray(g.nindex, dtype=np.int32)
+    nl = np.ascontiguousarray(g.nlist, dtype=np.int32)
+    ww = np.ascontiguousarray(w, dtype=np.int32)
+    out = np.empty(max(g.edges, 1), dtype=np.uint8)
+    L = N.mst_lib()
+    walls, dev = [], []
+    for k in range(calls + 1):
+        sec = C.c_double(0.0)
+        t0 = time.perf_counter()
+        N.check_mst(L.eclmst_compute_timed(g.nodes, g.edges, ni.ctypes.data, nl.ctypes.data, ww.ctypes.data,
+                                           out.ctypes.data, C.byref(sec)))
+        if k:
+            walls.append(time.perf_counter() - t0)
+            dev.append(sec.value)
+    wall = statistics.median(walls)
+    h2d = 4 * (g.nodes + 1) + 8 * g.edges
+    return {
+        "ms_per_call": 1e3 * wall,
+        "device_ms": 1e3 * statistics.median(dev),
+        "edges_per_s": g.edges / wall,
+        "h2d_bytes": h2d,
+        "d2h_bytes": g.edges,
+        "copy_GBps": (h2d + g.edges) / wall / 1e9,
+        "pcie_peak_GBps": 64.0,  # PCIe 5.0 x16, one direction
+        "pcie_frac": (h2d + g.edges) / wall / 64.0e9,
+        "forest_equal": bool(np.array_equal(out[: g.edges], forest)),
+        "calls": calls,
+    }
 
 
 def copy_probe(args) -> dict:

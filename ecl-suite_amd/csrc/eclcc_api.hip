@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "eclcc_internal.h"
+#include "eclcc_stage.h"
 
 namespace eclcc {
 
@@ -458,9 +459,10 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } sg{s};
   // host CSR in, both arrays in one pass through pinned staging
-  const CopySpan csr[2] = {{d_idx, const_cast<int32_t*>(nindex), sizeof(int32_t) * ((size_t)nodes + 1)},
-                           {d_lst, const_cast<int32_t*>(nlist), sizeof(int32_t) * (size_t)edges}};
-  if ((rc = upload_host(csr, 2, s))) return rc;
+  const eclstage::CopySpan csr[2] = {
+      {d_idx, const_cast<int32_t*>(nindex), sizeof(int32_t) * ((size_t)nodes + 1)},
+      {d_lst, const_cast<int32_t*>(nlist), sizeof(int32_t) * (size_t)edges}};
+  if ((rc = check_hip(eclstage::to_device(csr, 2, s), ECLCC_ERR_COPY, "copying to device failed"))) return rc;
   float kms[kNumKernels], tot = 0.f;
   rc = eclcc_run_device_profiled(nodes, edges, (const int32_t*)d_idx, (const int32_t*)d_lst,
                                  (int32_t*)d_st, d_w, wbytes, params, s, kms, &tot);
@@ -468,7 +470,8 @@ int eclcc_compute_timed(int32_t nodes, int32_t edges, const int32_t* nindex, con
   if (seconds) *seconds = 0.001 * (double)tot;
   // labels out through pinned staging too: a pageable copy back to the host
   // ran at about 7 GB/s (64 MiB in 9.5 ms)
-  return download_host(nstat, d_st, sizeof(int32_t) * (size_t)nodes, s);
+  return check_hip(eclstage::to_host(nstat, d_st, sizeof(int32_t) * (size_t)nodes, s), ECLCC_ERR_COPY,
+                   "copying from device failed");
 } catch (...) {
   return host_exception();
 }

@@ -130,18 +130,6 @@ hipError_t launch_init_only(const KParams& p, hipStream_t s);
 // Blocks of the block-level kernel one CU holds at once (occupancy API).
 int hub_block_residency(int block_threads);
 
-// Copies between pageable host memory and the device, ordered after the
-// work already on `s` (eclcc_upload.hip).  Large copies go through pinned
-// staging slots on several streams and are complete on return; so is every
-// download.  upload_host takes several arrays in one staged pass.
-struct CopySpan {
-  void* dev;
-  void* host;
-  size_t bytes;
-};
-int upload_host(const CopySpan* spans, int nspans, hipStream_t s);
-int upload_host(void* dst, const void* src, size_t bytes, hipStream_t s);
-int download_host(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Status helpers (thread-local message).
 int set_error(int code, const std::string& msg);

@@ -1,7 +1,9 @@
-// eclcc_upload.hip -- copies between large pageable host arrays and the
-// device for the drop-in call (eclcc_compute), which takes the caller's host
-// CSR and fills the caller's host labels as the reference's computeCC does
-// (ECL-CC-atomic.cu:295-296, 318).
+// eclcc_stage.hip -- copies between large pageable host arrays and the
+// device for the drop-in calls, which take the caller's host arrays and fill
+// the caller's host output as the reference's computeCC and gpuMST do
+// (ECL-CC-atomic.cu:295-296, 318; ECL-MST_10-atomic.cu:292-297, 365).  Built
+// into libeclcc.so and libeclmst.so, each with a pool of its own (the
+// functions are hidden, eclcc_stage.h).
 //
 // On the MI355X box a copy from pageable memory is staged by the runtime and
 // ran at 27-30 GB/s (2.08 GB of R-MAT-24 entries in 70-76 ms), a copy from
@@ -26,9 +28,9 @@
 #include <thread>
 #include <vector>
 
-#include "eclcc_internal.h"
+#include "eclcc_stage.h"
 
-namespace eclcc {
+namespace eclstage {
 namespace {
 
 constexpr size_t kStageChunk = size_t(8) << 20;
@@ -101,21 +103,19 @@ void note(std::atomic<int>& err, hipError_t e) {
 }
 
 // Runs worker(t) on kStageThreads threads (this one included); the first
-// error any of them noted, as a status.
+// error any of them noted.
 template <class F>
-int run_workers(std::atomic<int>& err, F worker) {
+hipError_t run_workers(std::atomic<int>& err, F worker) {
   std::vector<std::thread> th;
-  th.reserve(kStageThreads - 1);
   try {
+    th.reserve(kStageThreads - 1);
     for (int t = 1; t < kStageThreads; t++) th.emplace_back(worker, t);
   } catch (...) {
     err.store(hipErrorOutOfMemory);  // the started threads stop at their next chunk
   }
   worker(0);
   for (std::thread& x : th) x.join();
-  if (err.load() != hipSuccess)
-    return check_hip((hipError_t)err.load(), ECLCC_ERR_COPY, "copying between host and device failed");
-  return ECLCC_OK;
+  return (hipError_t)err.load();
 }
 
 // Locks and makes the pool for a copy of `bytes` (total) at least `min`
@@ -131,25 +131,23 @@ bool take_pool(StagePool& p, std::unique_lock<std::mutex>& lock, size_t bytes, s
 
 }  // namespace
 
-int upload_host(const CopySpan* spans, int nspans, hipStream_t s) {
+hipError_t to_device(const CopySpan* spans, int nspans, hipStream_t s) {
   size_t total = 0;
   for (int i = 0; i < nspans; i++) total += spans[i].bytes;
-  if (total == 0) return ECLCC_OK;
+  if (total == 0) return hipSuccess;
   StagePool& p = pool();
   std::unique_lock<std::mutex> lock(p.mu, std::defer_lock);
   if (!take_pool(p, lock, total, kStageMinBytes)) {
     for (int i = 0; i < nspans; i++)
       if (spans[i].bytes) {
-        const int rc = check_hip(hipMemcpyAsync(spans[i].dev, spans[i].host, spans[i].bytes,
-                                                hipMemcpyHostToDevice, s),
-                                 ECLCC_ERR_COPY, "copying to device failed");
-        if (rc) return rc;
+        const hipError_t e = hipMemcpyAsync(spans[i].dev, spans[i].host, spans[i].bytes, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
       }
-    return ECLCC_OK;
+    return hipSuccess;
   }
   // work queued on `s` before the call finishes before the staged copies
-  int rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying to device failed");
-  if (rc) return rc;
+  const hipError_t e0 = hipStreamSynchronize(s);
+  if (e0 != hipSuccess) return e0;
   const std::vector<Chunk> chunks = cut(spans, nspans);
   std::atomic<int> err{hipSuccess};
   return run_workers(err, [&](int t) {
@@ -170,24 +168,17 @@ int upload_host(const CopySpan* spans, int nspans, hipStream_t s) {
   });
 }
 
-int upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  const CopySpan one{dst, const_cast<void*>(src), bytes};
-  return upload_host(&one, 1, s);
-}
-
-int download_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if (bytes == 0) return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+hipError_t to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipStreamSynchronize(s);
   StagePool& p = pool();
   std::unique_lock<std::mutex> lock(p.mu, std::defer_lock);
   if (!take_pool(p, lock, bytes, kStageMinDownBytes)) {
-    const int rc = check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), ECLCC_ERR_COPY,
-                             "copying from device failed");
-    if (rc) return rc;
-    return check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
   }
   // the data is written by work on `s`
-  int rc = check_hip(hipStreamSynchronize(s), ECLCC_ERR_COPY, "copying from device failed");
-  if (rc) return rc;
+  const hipError_t e0 = hipStreamSynchronize(s);
+  if (e0 != hipSuccess) return e0;
   const CopySpan one{const_cast<void*>(src), dst, bytes};
   const std::vector<Chunk> chunks = cut(&one, 1);
   std::atomic<int> err{hipSuccess};
@@ -217,4 +208,4 @@ int download_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
   });
 }
 
-}  // namespace eclcc
+}  // namespace eclstage

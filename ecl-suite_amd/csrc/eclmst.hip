@@ -46,6 +46,7 @@
 #include <string>
 
 #include "../../include/eclmst.h"
+#include "eclcc_stage.h"
 
 namespace eclmst {
 
@@ -969,19 +970,17 @@ static int compute(int32_t nodes, int32_t edges, const int32_t* nindex, const in
   }
   rc = hip_fail(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), ECLMST_ERR_DEVICE,
                 "stream creation failed");
-  const hipMemcpyKind h2d = hipMemcpyHostToDevice;
-  if (!rc) rc = hip_fail(hipMemcpyAsync(d_ni, nindex, 4 * ((size_t)nodes + 1), h2d, s), ECLMST_ERR_COPY, "copying to device failed");
-  if (!rc && edges > 0) {
-    rc = hip_fail(hipMemcpyAsync(d_nl, nlist, 4 * (size_t)edges, h2d, s), ECLMST_ERR_COPY, "copying to device failed");
-    if (!rc) rc = hip_fail(hipMemcpyAsync(d_w, eweight, 4 * (size_t)edges, h2d, s), ECLMST_ERR_COPY, "copying to device failed");
-  }
+  // the three host arrays in one pass through pinned staging (eclcc_stage.hip)
+  const eclstage::CopySpan in[3] = {{d_ni, const_cast<int32_t*>(nindex), 4 * ((size_t)nodes + 1)},
+                                    {d_nl, const_cast<int32_t*>(nlist), 4 * (size_t)edges},
+                                    {d_w, const_cast<int32_t*>(eweight), 4 * (size_t)edges}};
+  if (!rc) rc = hip_fail(eclstage::to_device(in, 3, s), ECLMST_ERR_COPY, "copying to device failed");
   const int32_t thr = eclmst_threshold(nodes, edges, eweight);
   float ms = 0.f;
   if (!rc) rc = run_device(nodes, edges, d_ni, d_nl, d_w, d_out, d_work, wb, thr, s, &ms);
-  if (!rc && edges > 0)
-    rc = hip_fail(hipMemcpyAsync(in_mst, d_out, (size_t)edges, hipMemcpyDeviceToHost, s), ECLMST_ERR_COPY,
-                  "copying from device failed");
   if (!rc) rc = hip_fail(hipStreamSynchronize(s), ECLMST_ERR_LAUNCH, "kernel execution failed");
+  if (!rc && edges > 0)
+    rc = hip_fail(eclstage::to_host(in_mst, d_out, (size_t)edges, s), ECLMST_ERR_COPY, "copying from device failed");
   if (seconds) *seconds = 1e-3 * ms;
   cleanup();
   return rc;
