@@ -66,20 +66,23 @@ class Server:
     def submit_device_many(self, reqs) -> list:
         """Submits device-resident requests together: `reqs` is a sequence of
         (nodes, edges, nindex, nlist, labels) as for submit_device."""
+        import torch
+
         reqs = list(reqs)
         k = len(reqs)
-        for nodes, edges, ni, nl, lab in reqs:
-            for t, need in ((ni, nodes + 1), (nl, edges), (lab, nodes)):
-                if not t.is_cuda or not t.is_contiguous() or str(t.dtype) != "torch.int32" \
-                        or t.numel() < need:
+        i32 = torch.int32
+        ptrs = np.zeros((3, k), dtype=np.uint64)
+        # one pass, the checks in the order of their cost (a bulk submit of
+        # a thousand requests spent about a millisecond here)
+        for i, (nodes, edges, ni, nl, lab) in enumerate(reqs):
+            for col, (t, need) in enumerate(((ni, nodes + 1), (nl, edges), (lab, nodes))):
+                if t.dtype is not i32 or not t.is_cuda or t.numel() < need or not t.is_contiguous():
                     raise ValueError("nindex/nlist/labels must be contiguous int32 device tensors "
                                      "of at least nodes+1 / edges / nodes entries")
+                ptrs[col, i] = t.data_ptr() if (col != 1 or edges) else 0
         nodes = np.array([r[0] for r in reqs], dtype=np.int32)
         edges = np.array([r[1] for r in reqs], dtype=np.int32)
-        ptr = lambda i, col: reqs[i][col].data_ptr() if (col != 3 or reqs[i][1]) else 0
-        ni = np.array([ptr(i, 2) for i in range(k)], dtype=np.uint64)
-        nl = np.array([ptr(i, 3) for i in range(k)], dtype=np.uint64)
-        lb = np.array([ptr(i, 4) for i in range(k)], dtype=np.uint64)
+        ni, nl, lb = ptrs[0].copy(), ptrs[1].copy(), ptrs[2].copy()
         ids = np.zeros(k, dtype=np.int64)
         N.check_cc(N.cc_lib().eclcc_server_submit_device_many(
             self._h, k, nodes.ctypes.data, edges.ctypes.data, ni.ctypes.data, nl.ctypes.data,
